@@ -1,0 +1,263 @@
+/*
+ * stkfmm_p2p.h -- C ABI of the MI355X (gfx950) P2P near-field library.
+ *
+ * Drop-in for STKFMM's point-to-point kernel evaluation path.  All entry points are extern "C",
+ * take plain pointers and sizes, and return a status code (0 = success, negative = error; the
+ * message is available from stkfmm_last_error()).  Reference interfaces each entry replaces are
+ * cited as file:line relative to the STKFMM source tree.
+ *
+ * Conventions (as in the reference):
+ *   - r = x_trg - x_src; coincident pairs (r2 == 0) contribute exactly 0;
+ *   - results are ADDED to the target buffer (STKFMM.hpp:95);
+ *   - the kernel scale factor (1/(4 pi) Laplace, 1/(8 pi) Stokes family) is applied;
+ *   - host-pointer entries use AoS layouts: coords [x0,y0,z0,x1,...], values kdim-interleaved.
+ *
+ * Thread safety: calls that share a plan, or host-pointer calls from several host threads at once,
+ * must be serialised by the caller (the reference object is not safe concurrently either,
+ * Stk3DFMM.cpp:90-108).
+ */
+#ifndef STKFMM_P2P_H_
+#define STKFMM_P2P_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- enumerations ------------------------------------------------------------------------- */
+
+/* stkfmm::KERNEL bit values, STKFMM_common.hpp:46-62 */
+enum stkfmm_kernel {
+    STKFMM_KERNEL_LapPGrad = 1,
+    STKFMM_KERNEL_LapPGradGrad = 2,
+    STKFMM_KERNEL_LapQPGradGrad = 4,
+    STKFMM_KERNEL_Stokes = 8,
+    STKFMM_KERNEL_RPY = 16,
+    STKFMM_KERNEL_StokesRegVel = 32,
+    STKFMM_KERNEL_StokesRegVelOmega = 64,
+    STKFMM_KERNEL_PVel = 128,
+    STKFMM_KERNEL_PVelGrad = 256,
+    STKFMM_KERNEL_PVelLaplacian = 512,
+    STKFMM_KERNEL_Traction = 1024
+};
+
+/* stkfmm::PPKERNEL, STKFMM_common.hpp:37-41 */
+enum stkfmm_ppkernel { STKFMM_PPKERNEL_SLS2T = 1, STKFMM_PPKERNEL_DLS2T = 2, STKFMM_PPKERNEL_L2T = 4 };
+
+/* arithmetic mode of the HIP kernels (no reference counterpart: the reference is FP64 only) */
+enum stkfmm_precision {
+    STKFMM_PREC_FP64 = 0,  /* IEEE FP64 throughout (matches the reference formulas to ~1e-15) */
+    STKFMM_PREC_MIXED = 1  /* FP64 displacement + accumulation, FP32 rsqrt (+1 Newton) and polynomial */
+};
+
+/* micro-kernels: one per uKerEval of the reference (pvfmm::Kernel<double>::Ker_t entries) */
+enum stkfmm_uker {
+    STKFMM_UKER_LAPLACE_P = 0,               /* laplace_p               1 -> 1   LaplaceLayerKernel.hpp:41-65   */
+    STKFMM_UKER_LAPLACE_DIPOLEP = 1,         /* laplace_dipolep         3 -> 1   LaplaceLayerKernel.hpp:98-113  */
+    STKFMM_UKER_LAPLACE_PGRAD = 2,           /* laplace_pgrad           1 -> 4   LaplaceLayerKernel.hpp:67-96   */
+    STKFMM_UKER_LAPLACE_DIPOLEPGRAD = 3,     /* laplace_dipolepgrad     3 -> 4   LaplaceLayerKernel.hpp:115-136 */
+    STKFMM_UKER_LAPLACE_PGRADGRAD = 4,       /* laplace_pgradgrad       1 -> 10  LaplaceLayerKernel.hpp:138-166 */
+    STKFMM_UKER_LAPLACE_DIPOLEPGRADGRAD = 5, /* laplace_dipolepgradgrad 3 -> 10  LaplaceLayerKernel.hpp:168-200 */
+    STKFMM_UKER_STOKES_VEL = 6,              /* PVFMM stokes_vel        3 -> 3   StokesLayerKernel.hpp:38-42    */
+    STKFMM_UKER_STOKES_PVEL = 7,             /* stokes_pvel             4 -> 4   StokesSingleLayerKernel.hpp:27-47 */
+    STKFMM_UKER_STOKES_DOUBLEPVEL = 8,       /* stokes_doublepvel       9 -> 4   StokesDoubleLayerKernel.hpp:25-57 */
+    STKFMM_UKER_STOKES_REGVEL = 9,           /* stokes_regvel           4 -> 3   StokesRegSingleLayerKernel.hpp:17-41 */
+    STKFMM_UKER_RPY_U = 10,                  /* rpy_u                   4 -> 3   RPYKernel.hpp:17-45 */
+    STKFMM_UKER_RPY_ULAPU = 11,              /* rpy_ulapu               4 -> 6   RPYKernel.hpp:52-90 */
+    STKFMM_UKER_STK_ULAPU = 12,              /* stk_ulapu               3 -> 6   RPYKernel.hpp:97-129 */
+    STKFMM_UKER_STOKES_PVEL_FUSED = 13,      /* stokes_pvel + stokes_doublepvel at one point, 4+9 -> 4 */
+    STKFMM_UKER_COUNT = 14
+};
+
+/* status codes */
+enum stkfmm_status {
+    STKFMM_OK = 0,
+    STKFMM_ERR_KERNEL = -1,      /* unknown KERNEL (reference: exit(1), STKFMM.cpp:53-55) */
+    STKFMM_ERR_PPKERNEL = -2,    /* sub-kernel absent (reference: message + return, FMMData.cpp:270-273) */
+    STKFMM_ERR_ARG = -3,         /* bad size, pointer or layout */
+    STKFMM_ERR_HIP = -4,         /* HIP runtime error */
+    STKFMM_ERR_UNSUPPORTED = -5, /* valid request this build does not implement (e.g. dof != 1) */
+    STKFMM_ERR_INACTIVE = -6     /* kernel not active in the object (reference: exit(1), STKFMM.cpp:92-95) */
+};
+
+/* last error message of the calling thread ("" when none) */
+const char *stkfmm_last_error(void);
+
+/* ---- kernel table ------------------------------------------------------------------------- */
+
+/* dims = {kdimSL, kdimDL, kdimTrg}.  Replaces Stk3DFMM_get_kernel_dimension (Stk3DFMM-c.cpp:18-20)
+ * and stkfmm::getKernelDimension (STKFMM.cpp:33-40). */
+int stkfmm_get_kernel_dimension(unsigned kernel, int *dims);
+
+/* kernel name as ker_name (STKFMM.cpp:42-46); buf receives a NUL-terminated string */
+int stkfmm_get_kernel_name(unsigned kernel, char *buf, int buflen);
+
+/* micro-kernel a (KERNEL, PPKERNEL) pair evaluates: the k_s2t->ker_poten / k_s2t->dbl_layer_poten /
+ * k_l2t->ker_poten selection of FMMData::evaluateKernel (FMMData.cpp:261-268). */
+int stkfmm_get_uker(unsigned kernel, unsigned p2p, int *uker);
+
+/* source and target dimension of a micro-kernel */
+int stkfmm_uker_dims(int uker, int *kdim_src, int *kdim_trg);
+
+/* ---- boundary 1: direct O(N^2) evaluation ------------------------------------------------- */
+
+/* Replaces stkfmm::STKFMM::evaluateKernel (STKFMM.hpp:108-110, STKFMM.cpp:88-99, body
+ * FMMData.cpp:252-283).  Host AoS buffers; trgValue += result.  nThreads is accepted for
+ * signature compatibility and ignored (the work runs on the current HIP device).  FP64. */
+int stkfmm_evaluate_kernel(unsigned kernel, int nThreads, unsigned p2p, int nSrc, const double *srcCoord,
+                           const double *srcValue, int nTrg, const double *trgCoord, double *trgValue);
+
+/* Same as stkfmm_evaluate_kernel with a precision choice. */
+int stkfmm_p2p_eval(unsigned kernel, unsigned p2p, int precision, int nSrc, const double *srcCoord,
+                    const double *srcValue, int nTrg, const double *trgCoord, double *trgValue);
+
+/* Device-resident variant: AoS device buffers, enqueued on `stream` (hipStream_t, 0 = null stream),
+ * asynchronous.  No host memory is touched. */
+int stkfmm_p2p_eval_device(unsigned kernel, unsigned p2p, int precision, int nSrc, const double *d_srcCoord,
+                           const double *d_srcValue, int nTrg, const double *d_trgCoord, double *d_trgValue,
+                           void *stream);
+
+/* ---- boundary 2: pvfmm::Kernel<double>::Ker_t micro-kernel entries ------------------------- */
+
+/* Each has the Ker_t signature (r_src, src_cnt, v_src, dof, r_trg, trg_cnt, v_trg, mem_mgr) whose
+ * arity and argument order are read off the calls at FMMData.cpp:261,280-281; the typedef itself is
+ * PVFMM's (not vendored).  Host AoS buffers, v_trg += ScaleFactor * sum, FP64, dof must be 1 (the
+ * only value STKFMM passes).  mem_mgr is ignored.  Errors are reported through stkfmm_last_error()
+ * because Ker_t returns void. */
+void stkfmm_ker_laplace_p(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg, int trg_cnt,
+                          double *v_trg, void *mem_mgr);
+void stkfmm_ker_laplace_dipolep(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg, int trg_cnt,
+                                double *v_trg, void *mem_mgr);
+void stkfmm_ker_laplace_pgrad(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg, int trg_cnt,
+                              double *v_trg, void *mem_mgr);
+void stkfmm_ker_laplace_dipolepgrad(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg,
+                                    int trg_cnt, double *v_trg, void *mem_mgr);
+void stkfmm_ker_laplace_pgradgrad(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg,
+                                  int trg_cnt, double *v_trg, void *mem_mgr);
+void stkfmm_ker_laplace_dipolepgradgrad(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg,
+                                        int trg_cnt, double *v_trg, void *mem_mgr);
+void stkfmm_ker_stokes_vel(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg, int trg_cnt,
+                           double *v_trg, void *mem_mgr);
+void stkfmm_ker_stokes_pvel(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg, int trg_cnt,
+                            double *v_trg, void *mem_mgr);
+void stkfmm_ker_stokes_doublepvel(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg,
+                                  int trg_cnt, double *v_trg, void *mem_mgr);
+void stkfmm_ker_stokes_regvel(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg, int trg_cnt,
+                              double *v_trg, void *mem_mgr);
+void stkfmm_ker_rpy_u(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg, int trg_cnt,
+                      double *v_trg, void *mem_mgr);
+void stkfmm_ker_rpy_ulapu(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg, int trg_cnt,
+                          double *v_trg, void *mem_mgr);
+void stkfmm_ker_stk_ulapu(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg, int trg_cnt,
+                          double *v_trg, void *mem_mgr);
+
+/* generic form of the entries above, status-returning */
+int stkfmm_ker_eval(int uker, const double *r_src, int src_cnt, const double *v_src, int dof,
+                    const double *r_trg, int trg_cnt, double *v_trg);
+
+/* ---- batched evaluation over a work list ---------------------------------------------------- */
+
+/* Strided view of a device array of points: element (i, k) is ptr[i*stride + k*fstride].
+ * AoS: stride = kdim, fstride = 1.  SoA: stride = 1, fstride = n. */
+typedef struct stkfmm_array {
+    const double *ptr;
+    long long stride;
+    long long fstride;
+} stkfmm_array;
+
+typedef struct stkfmm_out_array {
+    double *ptr;
+    long long stride;
+    long long fstride;
+} stkfmm_out_array;
+
+/* Tuning knobs (all optional: 0 means "library default / autotuned value"). */
+typedef struct stkfmm_tune {
+    int targets_per_lane; /* 1, 2 or 4 targets held by each lane */
+    int max_src_per_item; /* sources per work item before a target tile's list is split */
+    int persistent;       /* 1: persistent waves pull items from a cost-sorted queue (default);
+                             2: one launch, one workgroup per item, in queue order;
+                             3: one launch per target group (per-leaf launches, no queue) */
+    int waves_per_cu;     /* persistent grid = CUs * waves_per_cu (capped by occupancy) */
+} stkfmm_tune;
+
+typedef struct stkfmm_plan stkfmm_plan;
+
+/* Build a plan from a work list of target groups (U-list leaves or independent instances).
+ * Group g owns targets [grp_trg_begin[g], grp_trg_end[g]) and the source ranges
+ * r in [grp_rng_ptr[g], grp_rng_ptr[g+1]): sources [rng_src_begin[r], rng_src_end[r]).
+ * grp_level[g] (may be NULL) is the octree level of the group; in MIXED mode the pair arithmetic
+ * runs in coordinates scaled by 2^level (exact) and the outputs are rescaled by each component's
+ * homogeneity degree -- the scaleSrc/scaleTrg rule of FMMData.cpp:285-470.
+ * Every target index must lie in [0, n_trg) and every source index in [0, n_src); the arrays given to
+ * stkfmm_plan_execute must hold at least that many points (checked here, trusted there).
+ * Host arrays; the plan copies what it needs to the current device.  tune may be NULL. */
+int stkfmm_plan_create(stkfmm_plan **plan, int uker, int precision, int n_src, int n_trg, int n_groups,
+                       const int *grp_trg_begin, const int *grp_trg_end, const int *grp_level,
+                       const int *grp_rng_ptr, const int *rng_src_begin, const int *rng_src_end,
+                       const stkfmm_tune *tune);
+
+/* Evaluate the plan: trg_value(perm[i], :) += ScaleFactor * sum over the work list (perm may be
+ * NULL: identity).  src_value holds the first kdim values of each source; for
+ * STKFMM_UKER_STOKES_PVEL_FUSED src_value holds the 4 single-layer values and src_value2 the 9
+ * double-layer values of the same point (src_value2 is ignored for other micro-kernels).
+ * Device arrays, enqueued on `stream`, asynchronous. */
+int stkfmm_plan_execute(stkfmm_plan *plan, const stkfmm_array *src_coord, const stkfmm_array *src_value,
+                        const stkfmm_array *src_value2, const stkfmm_array *trg_coord,
+                        const stkfmm_out_array *trg_value, const int *d_trg_perm, void *stream);
+
+/* Capture plan_execute with these exact arguments into a HIP graph held by the plan; subsequent
+ * stkfmm_plan_launch_graph calls replay it.  Re-capturing replaces the previous graph. */
+int stkfmm_plan_capture_graph(stkfmm_plan *plan, const stkfmm_array *src_coord, const stkfmm_array *src_value,
+                              const stkfmm_array *src_value2, const stkfmm_array *trg_coord,
+                              const stkfmm_out_array *trg_value, const int *d_trg_perm, void *stream);
+int stkfmm_plan_launch_graph(stkfmm_plan *plan, void *stream);
+
+/* statistics: pairs = sum over groups of n_trg * n_src; n_items = work items; n_tiles = target tiles */
+int stkfmm_plan_stats(const stkfmm_plan *plan, long long *pairs, int *n_items, int *n_tiles, int *grid);
+
+/* Kernel timing with HIP events on the execution stream.  set_timing(plan, 1) starts recording
+ * (and clears earlier records); each later stkfmm_plan_execute records the duration of its item
+ * kernel(s) and of its reduction.  kernel_ms returns their averages over the recorded executes (the
+ * last 256) and how many there were; the stream must have been synchronised. */
+int stkfmm_plan_set_timing(stkfmm_plan *plan, int enable);
+int stkfmm_plan_kernel_ms(const stkfmm_plan *plan, float *item_ms, float *reduce_ms, int *count);
+
+int stkfmm_plan_destroy(stkfmm_plan *plan);
+
+/* ---- adaptive octree and U-list (host) ------------------------------------------------------ */
+
+/* Adaptive octree over points in [0,1)^3: a box is split while it holds more than leaf_cap points
+ * (the maxPts rule of FMMData::setupTree, FMMData.cpp:111-113).  U-list of leaf B = every leaf whose
+ * closed box touches B's (any level, B included).  Points are sorted along the Morton curve; each
+ * leaf is a contiguous range of the sorted order. */
+typedef struct stkfmm_tree stkfmm_tree;
+int stkfmm_tree_build(stkfmm_tree **tree, int n, const double *coord, int leaf_cap);
+/* sizes: leaves, U-list source ranges (after merging Morton-adjacent leaves), deepest level, pairs */
+int stkfmm_tree_info(const stkfmm_tree *tree, int *n_leaves, int *n_ranges, int *max_level, long long *pairs);
+/* perm[i] = original index of sorted point i; leaf arrays [n_leaves]; ulist_ptr [n_leaves+1];
+ * rng_begin/rng_end [n_ranges] (sorted-order indices) */
+int stkfmm_tree_get(const stkfmm_tree *tree, int *perm, int *leaf_begin, int *leaf_end, int *leaf_level,
+                    int *ulist_ptr, int *rng_begin, int *rng_end);
+int stkfmm_tree_destroy(stkfmm_tree *tree);
+
+/* ---- adjoints ----------------------------------------------------------------------------- */
+
+/* Gradients of L = sum_t w_t . u_t for u = direct-sum P2P (same conventions as
+ * stkfmm_p2p_eval_device), for STKFMM_UKER_STOKES_PVEL, STKFMM_UKER_STOKES_DOUBLEPVEL,
+ * STKFMM_UKER_STOKES_PVEL_FUSED and STKFMM_UKER_RPY_ULAPU.  No reference counterpart (the reference
+ * has no derivatives); callers: the torch.autograd.Function in the Python binding.
+ * Device AoS arrays.  Outputs are OVERWRITTEN: g_src_value [nSrc*ks], g_src_coord [nSrc*3],
+ * g_trg_coord [nTrg*3]; any of them may be NULL to skip it.  For the fused kernel src_value holds
+ * the 13 values [f(3), TrD, D(9)] of each point. */
+int stkfmm_p2p_vjp_device(int uker, int precision, int nSrc, const double *d_srcCoord, const double *d_srcValue,
+                          int nTrg, const double *d_trgCoord, const double *d_w, double *d_g_src_value,
+                          double *d_g_src_coord, double *d_g_trg_coord, void *stream);
+
+/* ---- device information ------------------------------------------------------------------- */
+int stkfmm_device_info(int *n_cu, int *warp_size, char *arch, int arch_len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STKFMM_P2P_H_ */

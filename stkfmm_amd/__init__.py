@@ -1,0 +1,472 @@
+"""stkfmm_amd -- MI355X-native (gfx950 HIP) P2P near-field kernels of STKFMM.
+
+Host-side mirror of the reference's P2P interface over the C ABI of ``libstkfmm_p2p.so``
+(``include/stkfmm_p2p.h``):
+
+* enums ``PAXIS``, ``KERNEL`` (as ``Python/PySTKFMM.py:11-31``), ``PPKERNEL``
+  (``Lib/include/STKFMM/STKFMM_common.hpp:37-41``), plus ``PREC`` and ``UKER``;
+* ``get_kernel_dimension`` / ``get_kernel_name`` (``Lib/src/STKFMM.cpp:33-46``);
+* ``evaluate_kernel`` and ``Stk3DFMM.evaluate_kernel`` -- ``STKFMM::evaluateKernel``
+  (``Lib/include/STKFMM/STKFMM.hpp:108-110``): host numpy AoS arrays, results added to ``trg_value``;
+* device entry points on torch tensors (``p2p_device``, ``uker_device``, ``vjp_device``), the
+  adaptive octree / U-list builder (``Tree``) and batched work-list plans (``Plan``).
+
+Every compute call goes to the HIP library; there is no CPU fallback.  torch is only used for device
+memory and streams.
+"""
+import ctypes
+import enum
+
+import numpy as np
+
+from . import _capi
+from ._capi import StkArray, StkOutArray, StkTune
+
+
+class PAXIS(enum.IntEnum):
+    NONE = 0
+    PX = 1
+    PXY = 2
+    PXYZ = 3
+
+
+class KERNEL(enum.IntFlag):
+    LapPGrad = 1
+    LapPGradGrad = 2
+    LapQPGradGrad = 4
+    Stokes = 8
+    RPY = 16
+    StokesRegVel = 32
+    StokesRegVelOmega = 64
+    PVel = 128
+    PVelGrad = 256
+    PVelLaplacian = 512
+    Traction = 1024
+
+
+class PPKERNEL(enum.IntEnum):
+    SLS2T = 1
+    DLS2T = 2
+    L2T = 4
+
+
+class PREC(enum.IntEnum):
+    FP64 = 0
+    MIXED = 1
+
+
+class UKER(enum.IntEnum):
+    LAPLACE_P = 0
+    LAPLACE_DIPOLEP = 1
+    LAPLACE_PGRAD = 2
+    LAPLACE_DIPOLEPGRAD = 3
+    LAPLACE_PGRADGRAD = 4
+    LAPLACE_DIPOLEPGRADGRAD = 5
+    STOKES_VEL = 6
+    STOKES_PVEL = 7
+    STOKES_DOUBLEPVEL = 8
+    STOKES_REGVEL = 9
+    RPY_U = 10
+    RPY_ULAPU = 11
+    STK_ULAPU = 12
+    STOKES_PVEL_FUSED = 13
+
+
+# ---- errors ---------------------------------------------------------------------------------------
+
+class STKFMMError(RuntimeError):
+    code = None
+
+
+class KernelNotFound(STKFMMError):
+    """Unknown KERNEL (the reference prints and calls exit(1), Lib/src/STKFMM.cpp:53-55)."""
+    code = -1
+
+
+class PPKernelNotFound(STKFMMError):
+    """Sub-kernel absent (the reference prints and returns, Lib/src/FMMData.cpp:270-273)."""
+    code = -2
+
+
+class ArgumentError(STKFMMError, ValueError):
+    code = -3
+
+
+class HIPError(STKFMMError):
+    code = -4
+
+
+class Unsupported(STKFMMError):
+    code = -5
+
+
+class KernelNotActive(STKFMMError):
+    """Kernel not active in the object (the reference prints and calls exit(1), STKFMM.cpp:92-95)."""
+    code = -6
+
+
+_ERRORS = {c.code: c for c in (KernelNotFound, PPKernelNotFound, ArgumentError, HIPError, Unsupported,
+                               KernelNotActive)}
+
+
+def _check(rc):
+    if rc != 0:
+        raise _ERRORS.get(rc, STKFMMError)(f"[{rc}] {_capi.last_error()}")
+
+
+def lib():
+    return _capi.lib()
+
+
+# ---- kernel table ---------------------------------------------------------------------------------
+
+def get_kernel_dimension(kernel):
+    """[kdimSL, kdimDL, kdimTrg] (Lib/src/STKFMM.cpp:33-40)."""
+    dims = np.zeros(3, dtype=np.int32)
+    _check(lib().stkfmm_get_kernel_dimension(int(kernel), dims.ctypes.data_as(_capi.iptr)))
+    return dims
+
+
+def get_kernel_name(kernel):
+    buf = ctypes.create_string_buffer(64)
+    _check(lib().stkfmm_get_kernel_name(int(kernel), buf, 64))
+    return buf.value.decode()
+
+
+def get_uker(kernel, p2p):
+    """Micro-kernel evaluated for (KERNEL, PPKERNEL) (Lib/src/FMMData.cpp:261-268)."""
+    uk = ctypes.c_int(-1)
+    _check(lib().stkfmm_get_uker(int(kernel), int(p2p), ctypes.byref(uk)))
+    return UKER(uk.value)
+
+
+def uker_dims(uker):
+    """(source dim, target dim) of a micro-kernel."""
+    ks, kt = ctypes.c_int(), ctypes.c_int()
+    _check(lib().stkfmm_uker_dims(int(uker), ctypes.byref(ks), ctypes.byref(kt)))
+    return ks.value, kt.value
+
+
+def device_info():
+    n_cu, ws = ctypes.c_int(), ctypes.c_int()
+    arch = ctypes.create_string_buffer(64)
+    _check(lib().stkfmm_device_info(ctypes.byref(n_cu), ctypes.byref(ws), arch, 64))
+    return dict(n_cu=n_cu.value, warp_size=ws.value, arch=arch.value.decode())
+
+
+# ---- host (numpy) entry: STKFMM::evaluateKernel ------------------------------------------------------
+
+def _host_arr(a, n, k, name, writable=False):
+    if not isinstance(a, np.ndarray) or a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise ArgumentError(f"{name}: expected a C-contiguous float64 numpy array")
+    if writable and not a.flags.writeable:
+        raise ArgumentError(f"{name}: array is read-only")
+    if a.size != n * k:
+        raise ArgumentError(f"{name}: expected {n}x{k} values, got {a.size}")
+    return a.ctypes.data_as(_capi.dptr)
+
+
+def evaluate_kernel(kernel, n_threads, p2p, src_coord, src_value, trg_coord, trg_value, precision=PREC.FP64):
+    """Direct O(N^2) P2P: ``trg_value += K(trg, src) src_value`` (STKFMM::evaluateKernel).
+
+    Arrays are host numpy float64, C-contiguous, AoS ([n,3] coords, [n,kdim] values).  The number
+    of points is taken from the coordinate arrays.  ``n_threads`` is ignored (GPU).
+    """
+    uk = get_uker(kernel, p2p)
+    ks, kt = uker_dims(uk)
+    ns = src_coord.size // 3
+    nt = trg_coord.size // 3
+    a = _host_arr(src_coord, ns, 3, "src_coord")
+    b = _host_arr(src_value, ns, ks, "src_value")
+    c = _host_arr(trg_coord, nt, 3, "trg_coord")
+    d = _host_arr(trg_value, nt, kt, "trg_value", writable=True)
+    _check(lib().stkfmm_p2p_eval(int(kernel), int(p2p), int(precision), ns, a, b, nt, c, d))
+    return trg_value
+
+
+class Stk3DFMM:
+    """P2P subset of ``Stk3DFMM`` (Python/PySTKFMM.py:34-80, Lib/include/STKFMM/STKFMM.hpp:30-110).
+
+    The FMM parts (set_box/set_points/setup_tree/evaluate_fmm) are the CPU PVFMM path and are not
+    part of this library; ``evaluate_kernel`` runs on the GPU.
+    """
+
+    def __init__(self, mult_order, max_pts, pbc, kernels):
+        self.mult_order = int(mult_order)
+        self.max_pts = int(max_pts)
+        self.pbc = PAXIS(int(pbc))
+        self.kernels = int(kernels)
+
+    def get_kernel_dimension(self, kernel):
+        return get_kernel_dimension(kernel)
+
+    def is_kernel_active(self, kernel):
+        return bool(self.kernels & int(kernel))
+
+    def show_active_kernels(self):
+        names = [get_kernel_name(k) for k in KERNEL if self.kernels & int(k)]
+        print("active kernels: " + "".join("\t" + n for n in names))
+
+    def evaluate_kernel(self, kernel, n_threads, p2p, src_coord, src_value, trg_coord, trg_value,
+                        precision=PREC.FP64):
+        if not self.is_kernel_active(kernel):
+            raise KernelNotActive(f"Error: no such FMMData exists for kernel {get_kernel_name(kernel)}")
+        return evaluate_kernel(kernel, n_threads, p2p, src_coord, src_value, trg_coord, trg_value, precision)
+
+
+def ker_eval(uker, r_src, v_src, r_trg, v_trg, dof=1):
+    """Ker_t-shaped micro-kernel entry on host arrays: v_trg += ScaleFactor * sum (FP64)."""
+    ks, kt = uker_dims(uker)
+    ns, nt = r_src.size // 3, r_trg.size // 3
+    a = _host_arr(r_src, ns, 3, "r_src")
+    b = _host_arr(v_src, ns, ks, "v_src")
+    c = _host_arr(r_trg, nt, 3, "r_trg")
+    d = _host_arr(v_trg, nt, kt, "v_trg", writable=True)
+    _check(lib().stkfmm_ker_eval(int(uker), a, ns, b, int(dof), c, nt, d))
+    return v_trg
+
+
+# ---- device (torch) entries ----------------------------------------------------------------------
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream_handle(stream):
+    torch = _torch()
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def _dev2d(t, n, k, name, writable=False):
+    """Validate a float64 CUDA tensor with at least n rows and k columns; returns the tensor."""
+    torch = _torch()
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float64:
+        raise ArgumentError(f"{name}: expected a float64 CUDA tensor")
+    if t.dim() != 2 or t.shape[0] < n or t.shape[1] < k:
+        raise ArgumentError(f"{name}: expected shape >= ({n}, {k}), got {tuple(t.shape)}")
+    if n > 0 and (t.stride(0) < 0 or t.stride(1) < 0):
+        raise ArgumentError(f"{name}: negative strides")
+    return t
+
+
+def _sarr(t):
+    return StkArray(t.data_ptr(), t.stride(0), t.stride(1))
+
+
+def _oarr(t):
+    return StkOutArray(t.data_ptr(), t.stride(0), t.stride(1))
+
+
+def p2p_device(kernel, p2p, src_coord, src_value, trg_coord, trg_value, precision=PREC.FP64, stream=None):
+    """Direct sum on device tensors (contiguous AoS [n,3], [n,ks], [nt,3], [nt,kt]); trg_value +=."""
+    uk = get_uker(kernel, p2p)
+    ks, kt = uker_dims(uk)
+    ns, nt = src_coord.shape[0], trg_coord.shape[0]
+    for t, n, k, nm in ((src_coord, ns, 3, "src_coord"), (src_value, ns, ks, "src_value"),
+                        (trg_coord, nt, 3, "trg_coord"), (trg_value, nt, kt, "trg_value")):
+        _dev2d(t, n, k, nm)
+        if not t.is_contiguous() or t.shape[1] != k or t.shape[0] != n:
+            raise ArgumentError(f"{nm}: expected a contiguous ({n}, {k}) tensor")
+    _check(lib().stkfmm_p2p_eval_device(int(kernel), int(p2p), int(precision), ns, src_coord.data_ptr(),
+                                        src_value.data_ptr(), nt, trg_coord.data_ptr(), trg_value.data_ptr(),
+                                        _stream_handle(stream)))
+    return trg_value
+
+
+_DIRECT_PLANS = {}
+
+
+def uker_device(uker, src_coord, src_value, trg_coord, trg_value, precision=PREC.FP64, stream=None):
+    """Direct sum of one micro-kernel on device tensors via a single-group plan; trg_value +=.
+
+    For ``UKER.STOKES_PVEL_FUSED`` src_value is [n, 13] = [f(3), TrD, D(9)].
+    """
+    ns, nt = src_coord.shape[0], trg_coord.shape[0]
+    key = (int(uker), int(precision), ns, nt, src_coord.device.index)
+    plan = _DIRECT_PLANS.get(key)
+    if plan is None:
+        if len(_DIRECT_PLANS) >= 16:
+            _DIRECT_PLANS.pop(next(iter(_DIRECT_PLANS)))
+        plan = _DIRECT_PLANS[key] = Plan(uker, precision, ns, nt, [0], [nt], None, [0, 1], [0], [ns])
+    if int(uker) == UKER.STOKES_PVEL_FUSED:
+        _dev2d(src_value, ns, 13, "src_value")
+        plan.execute(src_coord, src_value[:, :4], trg_coord, trg_value, src_value2=src_value[:, 4:],
+                     stream=stream)
+    else:
+        plan.execute(src_coord, src_value, trg_coord, trg_value, stream=stream)
+    return trg_value
+
+
+def vjp_device(uker, src_coord, src_value, trg_coord, w, precision=PREC.FP64, need=(True, True, True),
+               stream=None):
+    """Gradients of L = sum_t w_t . u_t for the direct sum u of ``uker``.
+
+    Returns (g_src_value [ns,ks], g_src_coord [ns,3], g_trg_coord [nt,3]); entries not asked for in
+    ``need`` are None.  Device tensors, contiguous AoS.
+    """
+    torch = _torch()
+    ks, kt = uker_dims(uker)
+    ns, nt = src_coord.shape[0], trg_coord.shape[0]
+    for t, n, k, nm in ((src_coord, ns, 3, "src_coord"), (src_value, ns, ks, "src_value"),
+                        (trg_coord, nt, 3, "trg_coord"), (w, nt, kt, "w")):
+        _dev2d(t, n, k, nm)
+        if not t.is_contiguous() or t.shape[1] != k or t.shape[0] != n:
+            raise ArgumentError(f"{nm}: expected a contiguous ({n}, {k}) tensor")
+    dev = src_coord.device
+    gsv = torch.empty((ns, ks), dtype=torch.float64, device=dev) if need[0] else None
+    gsc = torch.empty((ns, 3), dtype=torch.float64, device=dev) if need[1] else None
+    gtc = torch.empty((nt, 3), dtype=torch.float64, device=dev) if need[2] else None
+    p = lambda t: None if t is None else t.data_ptr()  # noqa: E731
+    _check(lib().stkfmm_p2p_vjp_device(int(uker), int(precision), ns, src_coord.data_ptr(), src_value.data_ptr(),
+                                       nt, trg_coord.data_ptr(), w.data_ptr(), p(gsv), p(gsc), p(gtc),
+                                       _stream_handle(stream)))
+    return gsv, gsc, gtc
+
+
+# ---- octree / U-list ---------------------------------------------------------------------------
+
+class Tree:
+    """Adaptive octree + U-list over points in [0,1)^3 (host).
+
+    Attributes (numpy int32): perm (sorted -> original index), leaf_begin, leaf_end, leaf_level,
+    ulist_ptr, rng_begin, rng_end; plus n_leaves, n_ranges, max_level, pairs.
+    """
+
+    def __init__(self, coord, leaf_cap):
+        coord = np.ascontiguousarray(coord, dtype=np.float64).reshape(-1, 3)
+        n = coord.shape[0]
+        h = ctypes.c_void_p()
+        _check(lib().stkfmm_tree_build(ctypes.byref(h), n, coord.ctypes.data_as(_capi.dptr), int(leaf_cap)))
+        try:
+            nl, nr, ml = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            pairs = ctypes.c_longlong()
+            _check(lib().stkfmm_tree_info(h, ctypes.byref(nl), ctypes.byref(nr), ctypes.byref(ml),
+                                          ctypes.byref(pairs)))
+            self.n = n
+            self.leaf_cap = int(leaf_cap)
+            self.n_leaves, self.n_ranges, self.max_level, self.pairs = nl.value, nr.value, ml.value, pairs.value
+            self.perm = np.zeros(n, dtype=np.int32)
+            self.leaf_begin = np.zeros(self.n_leaves, dtype=np.int32)
+            self.leaf_end = np.zeros(self.n_leaves, dtype=np.int32)
+            self.leaf_level = np.zeros(self.n_leaves, dtype=np.int32)
+            self.ulist_ptr = np.zeros(self.n_leaves + 1, dtype=np.int32)
+            self.rng_begin = np.zeros(max(self.n_ranges, 1), dtype=np.int32)
+            self.rng_end = np.zeros(max(self.n_ranges, 1), dtype=np.int32)
+            ip = lambda a: a.ctypes.data_as(_capi.iptr)  # noqa: E731
+            _check(lib().stkfmm_tree_get(h, ip(self.perm), ip(self.leaf_begin), ip(self.leaf_end),
+                                         ip(self.leaf_level), ip(self.ulist_ptr), ip(self.rng_begin),
+                                         ip(self.rng_end)))
+            self.rng_begin = self.rng_begin[: self.n_ranges]
+            self.rng_end = self.rng_end[: self.n_ranges]
+        finally:
+            lib().stkfmm_tree_destroy(h)
+
+
+# ---- batched plans ------------------------------------------------------------------------------
+
+class Plan:
+    """A batched P2P work list on the current device (``stkfmm_plan_*``).
+
+    Group g owns targets [grp_trg_begin[g], grp_trg_end[g]) and the source ranges
+    [rng_begin[r], rng_end[r]) for r in [grp_rng_ptr[g], grp_rng_ptr[g+1]).
+    """
+
+    def __init__(self, uker, precision, n_src, n_trg, grp_trg_begin, grp_trg_end, grp_level, grp_rng_ptr,
+                 rng_begin, rng_end, tune=None):
+        i32 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.int32)  # noqa: E731
+        gb, ge, gl, gp, rb, re = map(i32, (grp_trg_begin, grp_trg_end, grp_level, grp_rng_ptr, rng_begin,
+                                           rng_end))
+        ng = len(gb)
+        if len(ge) != ng or len(gp) != ng + 1 or (gl is not None and len(gl) != ng) or len(rb) != len(re):
+            raise ArgumentError("inconsistent work-list array lengths")
+        if ng and gp[-1] > len(rb):
+            raise ArgumentError("grp_rng_ptr points past the range arrays")
+        ip = lambda a: None if a is None else a.ctypes.data_as(_capi.iptr)  # noqa: E731
+        t = None
+        if tune is not None:
+            t = StkTune(int(tune.get("targets_per_lane", 0)), int(tune.get("max_src_per_item", 0)),
+                        int(tune.get("persistent", 0)), int(tune.get("waves_per_cu", 0)))
+        h = ctypes.c_void_p()
+        _check(lib().stkfmm_plan_create(ctypes.byref(h), int(uker), int(precision), int(n_src), int(n_trg), ng,
+                                        ip(gb), ip(ge), ip(gl), ip(gp), ip(rb), ip(re),
+                                        ctypes.byref(t) if t is not None else None))
+        self._h = h
+        self.uker = UKER(int(uker))
+        self.precision = PREC(int(precision))
+        self.n_src, self.n_trg = int(n_src), int(n_trg)
+        self.ks, self.kt = uker_dims(uker)
+        self._keep = None
+
+    @classmethod
+    def from_tree(cls, tree, uker, precision, tune=None):
+        """U-list plan over a Tree: every leaf is a target group against its U-list."""
+        return cls(uker, precision, tree.n, tree.n, tree.leaf_begin, tree.leaf_end, tree.leaf_level,
+                   tree.ulist_ptr, tree.rng_begin, tree.rng_end, tune)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                lib().stkfmm_plan_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def stats(self):
+        pairs = ctypes.c_longlong()
+        ni, nt, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().stkfmm_plan_stats(self._h, ctypes.byref(pairs), ctypes.byref(ni), ctypes.byref(nt),
+                                       ctypes.byref(g)))
+        return dict(pairs=pairs.value, n_items=ni.value, n_tiles=nt.value, grid=g.value)
+
+    def set_timing(self, enable=True):
+        _check(lib().stkfmm_plan_set_timing(self._h, 1 if enable else 0))
+
+    def kernel_ms(self):
+        """(average item-kernel ms, average reduce ms, count) over executes recorded since set_timing."""
+        a, b, n = ctypes.c_float(), ctypes.c_float(), ctypes.c_int()
+        _check(lib().stkfmm_plan_kernel_ms(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n)))
+        return a.value, b.value, n.value
+
+    def _args(self, src_coord, src_value, trg_coord, trg_value, src_value2, perm):
+        ks1 = 4 if self.uker == UKER.STOKES_PVEL_FUSED else self.ks
+        _dev2d(src_coord, self.n_src, 3, "src_coord")
+        _dev2d(src_value, self.n_src, ks1, "src_value")
+        _dev2d(trg_coord, self.n_trg, 3, "trg_coord")
+        if self.uker == UKER.STOKES_PVEL_FUSED:
+            if src_value2 is None:
+                raise ArgumentError("STOKES_PVEL_FUSED needs src_value2 (double-layer values)")
+            _dev2d(src_value2, self.n_src, 9, "src_value2")
+        if perm is not None:
+            torch = _torch()
+            if not perm.is_cuda or perm.dtype != torch.int32 or perm.numel() < self.n_trg:
+                raise ArgumentError("perm: expected an int32 CUDA tensor of n_trg entries")
+            n_out = trg_value.shape[0]
+            _dev2d(trg_value, 0, self.kt, "trg_value")
+            if self.n_trg and (int(perm[: self.n_trg].min()) < 0 or int(perm[: self.n_trg].max()) >= n_out):
+                raise ArgumentError("perm: values outside the output rows")
+        else:
+            _dev2d(trg_value, self.n_trg, self.kt, "trg_value")
+        sv2 = _sarr(src_value2) if src_value2 is not None else _sarr(src_value)
+        args = (_sarr(src_coord), _sarr(src_value), sv2, _sarr(trg_coord), _oarr(trg_value))
+        return args, (None if perm is None else perm.data_ptr())
+
+    def execute(self, src_coord, src_value, trg_coord, trg_value, src_value2=None, perm=None, stream=None):
+        """trg_value(perm[i]) += ScaleFactor * sum over the work list (device tensors, any 2-D strides)."""
+        (a, b, c, d, e), p = self._args(src_coord, src_value, trg_coord, trg_value, src_value2, perm)
+        _check(lib().stkfmm_plan_execute(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                         ctypes.byref(d), ctypes.byref(e), p, _stream_handle(stream)))
+        return trg_value
+
+    def capture_graph(self, src_coord, src_value, trg_coord, trg_value, src_value2=None, perm=None, stream=None):
+        """Capture execute(...) with these tensors into a HIP graph; replay with launch_graph()."""
+        (a, b, c, d, e), p = self._args(src_coord, src_value, trg_coord, trg_value, src_value2, perm)
+        _check(lib().stkfmm_plan_capture_graph(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                               ctypes.byref(d), ctypes.byref(e), p, _stream_handle(stream)))
+        self._keep = (src_coord, src_value, trg_coord, trg_value, src_value2, perm)
+
+    def launch_graph(self, stream=None):
+        _check(lib().stkfmm_plan_launch_graph(self._h, _stream_handle(stream)))

@@ -1,0 +1,101 @@
+"""ctypes declarations of libstkfmm_p2p.so (the C ABI in include/stkfmm_p2p.h).
+
+The library is loaded from this package directory; there is no fallback.  A missing library is an
+ImportError naming the build step, so nothing silently runs another code path.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libstkfmm_p2p.so")
+
+c_int, c_uint, c_double, c_void_p, c_longlong, c_float, c_char_p = (
+    ctypes.c_int, ctypes.c_uint, ctypes.c_double, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_float,
+    ctypes.c_char_p)
+dptr = ctypes.POINTER(ctypes.c_double)
+iptr = ctypes.POINTER(ctypes.c_int)
+
+
+class StkArray(ctypes.Structure):
+    """stkfmm_array: element (i, k) at ptr[i*stride + k*fstride] (device pointer)."""
+    _fields_ = [("ptr", c_void_p), ("stride", c_longlong), ("fstride", c_longlong)]
+
+
+class StkOutArray(ctypes.Structure):
+    _fields_ = [("ptr", c_void_p), ("stride", c_longlong), ("fstride", c_longlong)]
+
+
+class StkTune(ctypes.Structure):
+    _fields_ = [("targets_per_lane", c_int), ("max_src_per_item", c_int), ("persistent", c_int),
+                ("waves_per_cu", c_int)]
+
+
+KER_T_NAMES = ["laplace_p", "laplace_dipolep", "laplace_pgrad", "laplace_dipolepgrad", "laplace_pgradgrad",
+               "laplace_dipolepgradgrad", "stokes_vel", "stokes_pvel", "stokes_doublepvel", "stokes_regvel",
+               "rpy_u", "rpy_ulapu", "stk_ulapu"]
+
+# every symbol include/stkfmm_p2p.h declares
+EXPORTS = [
+    "stkfmm_last_error", "stkfmm_get_kernel_dimension", "stkfmm_get_kernel_name", "stkfmm_get_uker",
+    "stkfmm_uker_dims", "stkfmm_evaluate_kernel", "stkfmm_p2p_eval", "stkfmm_p2p_eval_device",
+    "stkfmm_ker_eval", "stkfmm_plan_create", "stkfmm_plan_execute", "stkfmm_plan_capture_graph",
+    "stkfmm_plan_launch_graph", "stkfmm_plan_stats", "stkfmm_plan_set_timing", "stkfmm_plan_kernel_ms",
+    "stkfmm_plan_destroy", "stkfmm_tree_build", "stkfmm_tree_info", "stkfmm_tree_get", "stkfmm_tree_destroy",
+    "stkfmm_p2p_vjp_device", "stkfmm_device_info",
+] + ["stkfmm_ker_" + n for n in KER_T_NAMES]
+
+
+def _declare(L):
+    L.stkfmm_last_error.restype = c_char_p
+    L.stkfmm_last_error.argtypes = []
+    L.stkfmm_get_kernel_dimension.argtypes = [c_uint, iptr]
+    L.stkfmm_get_kernel_name.argtypes = [c_uint, c_char_p, c_int]
+    L.stkfmm_get_uker.argtypes = [c_uint, c_uint, iptr]
+    L.stkfmm_uker_dims.argtypes = [c_int, iptr, iptr]
+    L.stkfmm_evaluate_kernel.argtypes = [c_uint, c_int, c_uint, c_int, dptr, dptr, c_int, dptr, dptr]
+    L.stkfmm_p2p_eval.argtypes = [c_uint, c_uint, c_int, c_int, dptr, dptr, c_int, dptr, dptr]
+    L.stkfmm_p2p_eval_device.argtypes = [c_uint, c_uint, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p,
+                                         c_void_p, c_void_p]
+    L.stkfmm_ker_eval.argtypes = [c_int, dptr, c_int, dptr, c_int, dptr, c_int, dptr]
+    for n in KER_T_NAMES:
+        f = getattr(L, "stkfmm_ker_" + n)
+        f.restype = None
+        f.argtypes = [dptr, c_int, dptr, c_int, dptr, c_int, dptr, c_void_p]
+    L.stkfmm_plan_create.argtypes = [ctypes.POINTER(c_void_p), c_int, c_int, c_int, c_int, c_int, iptr, iptr, iptr,
+                                     iptr, iptr, iptr, ctypes.POINTER(StkTune)]
+    arr = ctypes.POINTER(StkArray)
+    oarr = ctypes.POINTER(StkOutArray)
+    L.stkfmm_plan_execute.argtypes = [c_void_p, arr, arr, arr, arr, oarr, c_void_p, c_void_p]
+    L.stkfmm_plan_capture_graph.argtypes = [c_void_p, arr, arr, arr, arr, oarr, c_void_p, c_void_p]
+    L.stkfmm_plan_launch_graph.argtypes = [c_void_p, c_void_p]
+    L.stkfmm_plan_stats.argtypes = [c_void_p, ctypes.POINTER(c_longlong), iptr, iptr, iptr]
+    L.stkfmm_plan_set_timing.argtypes = [c_void_p, c_int]
+    L.stkfmm_plan_kernel_ms.argtypes = [c_void_p, ctypes.POINTER(c_float), ctypes.POINTER(c_float), iptr]
+    L.stkfmm_plan_destroy.argtypes = [c_void_p]
+    L.stkfmm_tree_build.argtypes = [ctypes.POINTER(c_void_p), c_int, dptr, c_int]
+    L.stkfmm_tree_info.argtypes = [c_void_p, iptr, iptr, iptr, ctypes.POINTER(c_longlong)]
+    L.stkfmm_tree_get.argtypes = [c_void_p, iptr, iptr, iptr, iptr, iptr, iptr, iptr]
+    L.stkfmm_tree_destroy.argtypes = [c_void_p]
+    L.stkfmm_p2p_vjp_device.argtypes = [c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                                        c_void_p, c_void_p, c_void_p, c_void_p]
+    L.stkfmm_device_info.argtypes = [iptr, iptr, c_char_p, c_int]
+    return L
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises ImportError when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"stkfmm_amd: native library {LIB_PATH} is missing; build it with "
+                "`python -c 'import __graft_entry__ as g; g.build()'` or `make -C stkfmm_amd/csrc`")
+        _lib = _declare(ctypes.CDLL(LIB_PATH))
+    return _lib
+
+
+def last_error():
+    return lib().stkfmm_last_error().decode()

@@ -1,0 +1,775 @@
+// p2p_capi.cpp -- host side of libstkfmm_p2p.so: the C ABI declared in include/stkfmm_p2p.h.
+//
+// Kernel table, work-list planning (tiling, chunking, cost-sorted queue), the direct-sum and
+// Ker_t-shaped entries, graph capture and the adjoint entry.  The arithmetic lives in
+// p2p_kernels.hip; nothing here computes a kernel value on the host.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <list>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/stkfmm_p2p.h"
+#include "p2p_internal.hpp"
+
+using namespace stkp2p;
+
+// ---- errors -------------------------------------------------------------------------------------
+
+static thread_local std::string g_last_error;
+
+static int set_error(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+static int hip_error(hipError_t e, const char *what) {
+    return set_error(STKFMM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(call, what)                                                                        \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return hip_error(e_, what);                                                            \
+    } while (0)
+
+extern "C" const char *stkfmm_last_error(void) { return g_last_error.c_str(); }
+
+// ---- kernel table (STKFMM.cpp:18-40, STKFMM_common.hpp:46-62, FMMData.cpp:261-268) ---------------
+
+namespace {
+struct KernelEntry {
+    unsigned kernel;
+    const char *name; // ker_name given to BuildKernel
+    int kdim_sl, kdim_dl, kdim_trg;
+    int uk_sl, uk_dl, uk_l2t; // -1: sub-kernel absent in the reference; -2: present but not built here
+};
+
+// kdimDL of kernels without a double layer is PVFMM's default surf_dim (PVFMM_COORD_DIM +
+// ker_dim[0]); PVFMM is not vendored, so those entries are inferred, not pinned.
+// L2T sub-kernels: LapPGrad/LapPGradGrad/PVel/Stokes use themselves (NULL in BuildKernel),
+// RPY uses stk_ulapu (RPYKernel.hpp:151), StokesRegVel uses PVFMM's Stokes velocity
+// (StokesRegSingleLayerKernel.hpp:212-214).
+const KernelEntry kTable[] = {
+    {1, "laplace_PGrad", 1, 3, 4, STKFMM_UKER_LAPLACE_PGRAD, STKFMM_UKER_LAPLACE_DIPOLEPGRAD, STKFMM_UKER_LAPLACE_PGRAD},
+    {2, "laplace_PGradGrad", 1, 3, 10, STKFMM_UKER_LAPLACE_PGRADGRAD, STKFMM_UKER_LAPLACE_DIPOLEPGRADGRAD,
+     STKFMM_UKER_LAPLACE_PGRADGRAD},
+    {4, "laplace_QPGradGrad", 9, 12, 10, -2, -1, STKFMM_UKER_LAPLACE_PGRADGRAD},
+    {8, "stokes_vel", 3, 6, 3, STKFMM_UKER_STOKES_VEL, -1, STKFMM_UKER_STOKES_VEL},
+    {16, "rpy_ulapu", 4, 7, 6, STKFMM_UKER_RPY_ULAPU, -1, STKFMM_UKER_STK_ULAPU},
+    {32, "stokes_regvel", 4, 7, 3, STKFMM_UKER_STOKES_REGVEL, -1, STKFMM_UKER_STOKES_VEL},
+    {64, "stokes_regftvelomega", 7, 10, 6, -2, -1, -2},
+    {128, "stokes_PVel", 4, 9, 4, STKFMM_UKER_STOKES_PVEL, STKFMM_UKER_STOKES_DOUBLEPVEL, STKFMM_UKER_STOKES_PVEL},
+    {256, "stokes_PVelGrad", 4, 9, 16, -2, -2, -2},
+    {512, "stokes_PVelLaplacian", 4, 9, 7, -2, -2, -2},
+    {1024, "stokes_Traction", 4, 9, 9, -2, -2, -2},
+};
+
+const KernelEntry *find_kernel(unsigned kernel) {
+    for (const auto &e : kTable)
+        if (e.kernel == kernel)
+            return &e;
+    return nullptr;
+}
+} // namespace
+
+extern "C" int stkfmm_get_kernel_dimension(unsigned kernel, int *dims) {
+    const KernelEntry *e = find_kernel(kernel);
+    if (!e)
+        return set_error(STKFMM_ERR_KERNEL, "Error: Kernel not found.");
+    if (!dims)
+        return set_error(STKFMM_ERR_ARG, "dims is NULL");
+    dims[0] = e->kdim_sl;
+    dims[1] = e->kdim_dl;
+    dims[2] = e->kdim_trg;
+    return STKFMM_OK;
+}
+
+extern "C" int stkfmm_get_kernel_name(unsigned kernel, char *buf, int buflen) {
+    const KernelEntry *e = find_kernel(kernel);
+    if (!e)
+        return set_error(STKFMM_ERR_KERNEL, "Error: Kernel not found.");
+    if (!buf || buflen <= 0)
+        return set_error(STKFMM_ERR_ARG, "buffer is NULL or empty");
+    snprintf(buf, (size_t)buflen, "%s", e->name);
+    return STKFMM_OK;
+}
+
+extern "C" int stkfmm_get_uker(unsigned kernel, unsigned p2p, int *uker) {
+    const KernelEntry *e = find_kernel(kernel);
+    if (!e)
+        return set_error(STKFMM_ERR_KERNEL, "Error: Kernel not found.");
+    int uk;
+    if (p2p == STKFMM_PPKERNEL_SLS2T)
+        uk = e->uk_sl;
+    else if (p2p == STKFMM_PPKERNEL_DLS2T)
+        uk = e->uk_dl;
+    else if (p2p == STKFMM_PPKERNEL_L2T)
+        uk = e->uk_l2t;
+    else
+        uk = -1;
+    if (uk == -1)
+        return set_error(STKFMM_ERR_PPKERNEL, "PPKernel %u not found for direct evaluation", p2p);
+    if (uk == -2)
+        return set_error(STKFMM_ERR_UNSUPPORTED, "kernel %s sub-kernel %u is not implemented in this library",
+                         e->name, p2p);
+    if (uker)
+        *uker = uk;
+    return STKFMM_OK;
+}
+
+extern "C" int stkfmm_uker_dims(int uker, int *kdim_src, int *kdim_trg) {
+    if (uker < 0 || uker >= STKFMM_UKER_COUNT)
+        return set_error(STKFMM_ERR_ARG, "unknown micro-kernel %d", uker);
+    if (kdim_src)
+        *kdim_src = uker_ks(uker);
+    if (kdim_trg)
+        *kdim_trg = uker_kt(uker);
+    return STKFMM_OK;
+}
+
+// ---- device info ----------------------------------------------------------------------------------
+
+static int device_cu_count(int *n_cu) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    HIP_TRY(hipDeviceGetAttribute(n_cu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    return STKFMM_OK;
+}
+
+extern "C" int stkfmm_device_info(int *n_cu, int *warp_size, char *arch, int arch_len) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    if (n_cu)
+        *n_cu = prop.multiProcessorCount;
+    if (warp_size)
+        *warp_size = prop.warpSize;
+    if (arch && arch_len > 0)
+        snprintf(arch, (size_t)arch_len, "%s", prop.gcnArchName);
+    return STKFMM_OK;
+}
+
+// ---- plans ------------------------------------------------------------------------------------------
+
+struct stkfmm_plan {
+    int uk = 0, prec = 0, T = 2, TT = 128, KS = 0, KS1 = 0, KT = 0;
+    int n_src = 0, n_trg = 0;
+    int n_items = 0, n_tiles = 0, n_ranges = 0;
+    int grid = 0, persistent = 1;
+    long long pairs = 0;
+    int device = 0;
+    Item *d_items = nullptr;
+    int2 *d_ranges = nullptr;
+    Tile *d_tiles = nullptr;
+    double *d_partial = nullptr;
+    int *d_counter = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    // timing ring: execute k records events ev[3*(k % kRing) + {0,1,2}] around the item and reduce
+    // kernels on the execution stream
+    static constexpr int kRing = 256;
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    long long n_timed = 0;
+    std::vector<int> group_items; // mode 3: item index ranges per launched group (host)
+};
+
+static void plan_free(stkfmm_plan *p) {
+    if (!p)
+        return;
+    if (p->gexec)
+        (void)hipGraphExecDestroy(p->gexec);
+    if (p->graph)
+        (void)hipGraphDestroy(p->graph);
+    for (auto &e : p->ev)
+        (void)hipEventDestroy(e);
+    (void)hipFree(p->d_items);
+    (void)hipFree(p->d_ranges);
+    (void)hipFree(p->d_tiles);
+    (void)hipFree(p->d_partial);
+    (void)hipFree(p->d_counter);
+    delete p;
+}
+
+template <class T>
+static int upload(T **dst, const std::vector<T> &src, const char *what) {
+    *dst = nullptr;
+    if (src.empty())
+        return STKFMM_OK;
+    HIP_TRY(hipMalloc((void **)dst, sizeof(T) * src.size()), what);
+    HIP_TRY(hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice), what);
+    return STKFMM_OK;
+}
+
+extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision, int n_src, int n_trg,
+                                  int n_groups, const int *grp_trg_begin, const int *grp_trg_end,
+                                  const int *grp_level, const int *grp_rng_ptr, const int *rng_src_begin,
+                                  const int *rng_src_end, const stkfmm_tune *tune) {
+    if (!plan_out)
+        return set_error(STKFMM_ERR_ARG, "plan pointer is NULL");
+    *plan_out = nullptr;
+    if (uk < 0 || uk >= STKFMM_UKER_COUNT)
+        return set_error(STKFMM_ERR_ARG, "unknown micro-kernel %d", uk);
+    if (precision != STKFMM_PREC_FP64 && precision != STKFMM_PREC_MIXED)
+        return set_error(STKFMM_ERR_ARG, "unknown precision %d", precision);
+    if (n_src < 0 || n_trg < 0 || n_groups < 0)
+        return set_error(STKFMM_ERR_ARG, "negative size");
+    if (n_groups > 0 && (!grp_trg_begin || !grp_trg_end || !grp_rng_ptr))
+        return set_error(STKFMM_ERR_ARG, "group arrays are NULL");
+
+    stkfmm_tune tn{0, 0, 0, 0};
+    if (tune)
+        tn = *tune;
+    const int T = tn.targets_per_lane ? tn.targets_per_lane : 2;
+    if (T != 1 && T != 2 && T != 4)
+        return set_error(STKFMM_ERR_ARG, "targets_per_lane must be 1, 2 or 4 (got %d)", T);
+    const int persistent = tn.persistent ? tn.persistent : 1;
+    if (persistent < 1 || persistent > 3)
+        return set_error(STKFMM_ERR_ARG, "persistent must be 1, 2 or 3 (got %d)", persistent);
+
+    // validate the work list
+    long long total_cost = 0; // sum over tiles of the tile's source count
+    const int TT = kWave * T;
+    for (int g = 0; g < n_groups; ++g) {
+        const int b = grp_trg_begin[g], e = grp_trg_end[g];
+        if (b < 0 || e < b || e > n_trg)
+            return set_error(STKFMM_ERR_ARG, "group %d: target range [%d,%d) outside [0,%d)", g, b, e, n_trg);
+        const int r0 = grp_rng_ptr[g], r1 = grp_rng_ptr[g + 1];
+        if (r0 < 0 || r1 < r0)
+            return set_error(STKFMM_ERR_ARG, "group %d: bad range pointers", g);
+        if (r1 > r0 && (!rng_src_begin || !rng_src_end))
+            return set_error(STKFMM_ERR_ARG, "range arrays are NULL");
+        long long ns = 0;
+        for (int r = r0; r < r1; ++r) {
+            const int sb = rng_src_begin[r], se = rng_src_end[r];
+            if (sb < 0 || se < sb || se > n_src)
+                return set_error(STKFMM_ERR_ARG, "range %d: sources [%d,%d) outside [0,%d)", r, sb, se, n_src);
+            ns += se - sb;
+        }
+        if (grp_level && (grp_level[g] < -60 || grp_level[g] > 60))
+            return set_error(STKFMM_ERR_ARG, "group %d: level %d out of range", g, grp_level[g]);
+        total_cost += ns * (long long)((e - b + TT - 1) / TT);
+    }
+
+    int n_cu = 0, bpc = 0;
+    int rc = device_cu_count(&n_cu);
+    if (rc)
+        return rc;
+    HIP_TRY(items_occupancy(uk, precision, T, &bpc), "occupancy query");
+    if (bpc < 1)
+        bpc = 1;
+    const int concurrent = n_cu * bpc;
+
+    long long max_src = tn.max_src_per_item;
+    if (max_src <= 0 && persistent == 3) {
+        max_src = 512; // per-group launches: split like a stand-alone direct sum of that group
+    } else if (max_src <= 0) {
+        const long long target_items = 4LL * concurrent;
+        max_src = (total_cost + target_items - 1) / std::max(1LL, target_items);
+        max_src = std::max(512LL, ((max_src + kSrcTile - 1) / kSrcTile) * kSrcTile);
+    }
+    if (max_src < 1)
+        max_src = 1;
+
+    // tiles and items
+    std::vector<Tile> tiles;
+    std::vector<Item> items;
+    std::vector<int2> ranges;
+    std::vector<int> group_items(1, 0);
+    long long pairs = 0;
+    for (int g = 0; g < n_groups; ++g) {
+        const int b = grp_trg_begin[g], e = grp_trg_end[g];
+        const int r0 = grp_rng_ptr[g], r1 = grp_rng_ptr[g + 1];
+        long long ns = 0;
+        for (int r = r0; r < r1; ++r)
+            ns += rng_src_end[r] - rng_src_begin[r];
+        pairs += ns * (long long)(e - b);
+        if (ns == 0 || e == b)
+            continue;
+        const int level = grp_level ? grp_level[g] : 0;
+        for (int tb = b; tb < e; tb += TT) {
+            Tile tl;
+            tl.trg_begin = tb;
+            tl.trg_count = std::min(TT, e - tb);
+            tl.slot_begin = (int)items.size();
+            Item cur{tl.trg_begin, tl.trg_count, (int)ranges.size(), (int)ranges.size(), 0, level, 0, 0};
+            long long cur_n = 0;
+            for (int r = r0; r < r1; ++r) {
+                int sb = rng_src_begin[r];
+                const int se = rng_src_end[r];
+                while (sb < se) {
+                    const int m = (int)std::min<long long>(se - sb, max_src - cur_n);
+                    ranges.push_back(make_int2(sb, sb + m));
+                    cur.rng_end = (int)ranges.size();
+                    cur_n += m;
+                    sb += m;
+                    if (cur_n == max_src) {
+                        cur.cost = (int)cur_n;
+                        cur.slot = (int)items.size();
+                        items.push_back(cur);
+                        cur.rng_begin = cur.rng_end = (int)ranges.size();
+                        cur_n = 0;
+                    }
+                }
+            }
+            if (cur_n > 0) {
+                cur.cost = (int)cur_n;
+                cur.slot = (int)items.size();
+                items.push_back(cur);
+            }
+            tl.slot_end = (int)items.size();
+            tiles.push_back(tl);
+        }
+        if ((int)items.size() > group_items.back())
+            group_items.push_back((int)items.size());
+    }
+    // cost-sorted queue (largest first); ties keep work-list order.  Mode 3 keeps the groups apart.
+    if (persistent != 3)
+        std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) { return x.cost > y.cost; });
+
+    stkfmm_plan *p = new stkfmm_plan();
+    p->uk = uk;
+    p->prec = precision;
+    p->T = T;
+    p->TT = TT;
+    p->KS = uker_ks(uk);
+    p->KS1 = uker_ks1(uk);
+    p->KT = uker_kt(uk);
+    p->n_src = n_src;
+    p->n_trg = n_trg;
+    p->n_items = (int)items.size();
+    p->n_tiles = (int)tiles.size();
+    p->n_ranges = (int)ranges.size();
+    p->persistent = persistent;
+    int wpc = tn.waves_per_cu > 0 ? std::min(tn.waves_per_cu, bpc) : bpc;
+    p->grid = (persistent == 1) ? (int)std::min<long long>(p->n_items, (long long)n_cu * wpc) : p->n_items;
+    if (persistent == 3)
+        p->group_items = group_items;
+    p->pairs = pairs;
+    (void)hipGetDevice(&p->device);
+
+    int r;
+    if ((r = upload(&p->d_items, items, "plan items")) || (r = upload(&p->d_ranges, ranges, "plan ranges")) ||
+        (r = upload(&p->d_tiles, tiles, "plan tiles"))) {
+        plan_free(p);
+        return r;
+    }
+    const size_t partial_elems = (size_t)p->n_items * p->KT * p->TT;
+    hipError_t e = hipSuccess;
+    if (partial_elems)
+        e = hipMalloc((void **)&p->d_partial, sizeof(double) * partial_elems);
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&p->d_counter, sizeof(int));
+    if (e != hipSuccess) {
+        plan_free(p);
+        return hip_error(e, "plan workspace");
+    }
+    *plan_out = p;
+    return STKFMM_OK;
+}
+
+static bool arr_ok(const stkfmm_array *a) { return a && a->ptr && a->stride >= 0 && a->fstride >= 0; }
+
+static int plan_enqueue(stkfmm_plan *p, const stkfmm_array *sc, const stkfmm_array *sv, const stkfmm_array *sv2,
+                        const stkfmm_array *tc, const stkfmm_out_array *tv, const int *perm, hipStream_t s,
+                        bool allow_timing) {
+    if (p->n_items == 0)
+        return STKFMM_OK;
+    if (!arr_ok(sc) || !arr_ok(sv) || !arr_ok(tc) || !tv || !tv->ptr)
+        return set_error(STKFMM_ERR_ARG, "NULL device array");
+    if (p->KS1 < p->KS && !arr_ok(sv2))
+        return set_error(STKFMM_ERR_ARG, "this micro-kernel needs the second source-value array");
+    ItemArgs a;
+    a.sc = DArr{sc->ptr, sc->stride, sc->fstride};
+    a.v1 = DArr{sv->ptr, sv->stride, sv->fstride};
+    a.v2 = (p->KS1 < p->KS) ? DArr{sv2->ptr, sv2->stride, sv2->fstride} : a.v1;
+    a.tc = DArr{tc->ptr, tc->stride, tc->fstride};
+    a.items = p->d_items;
+    a.ranges = p->d_ranges;
+    a.n_items = p->n_items;
+    a.counter = (p->persistent == 1) ? p->d_counter : nullptr;
+    a.partial = p->d_partial;
+    ReduceArgs ra;
+    ra.tiles = p->d_tiles;
+    ra.n_tiles = p->n_tiles;
+    ra.kt = p->KT;
+    ra.tt = p->TT;
+    ra.partial = p->d_partial;
+    ra.out = OArr{tv->ptr, tv->stride, tv->fstride};
+    ra.perm = perm;
+    ra.scale = uker_scale(p->uk);
+    const bool timed = allow_timing && p->timing;
+    hipEvent_t *ev = timed ? &p->ev[3 * (p->n_timed % stkfmm_plan::kRing)] : nullptr;
+    if (timed) {
+        ++p->n_timed;
+        HIP_TRY(hipEventRecord(ev[0], s), "event record");
+    }
+    if (p->persistent == 1)
+        HIP_TRY(launch_fill_int(p->d_counter, 0, s), "counter reset launch");
+    if (p->persistent == 3) {
+        // one launch per target group, in work-list order (the per-leaf-launch scheme)
+        for (size_t g = 0; g + 1 < p->group_items.size(); ++g) {
+            ItemArgs ag = a;
+            ag.items = p->d_items + p->group_items[g];
+            ag.n_items = p->group_items[g + 1] - p->group_items[g];
+            HIP_TRY(launch_items(p->uk, p->prec, p->T, ag.n_items, ag, s), "p2p items launch");
+        }
+    } else {
+        HIP_TRY(launch_items(p->uk, p->prec, p->T, p->grid, a, s), "p2p items launch");
+    }
+    if (timed)
+        HIP_TRY(hipEventRecord(ev[1], s), "event record");
+    HIP_TRY(launch_reduce(ra, s), "p2p reduce launch");
+    if (timed)
+        HIP_TRY(hipEventRecord(ev[2], s), "event record");
+    return STKFMM_OK;
+}
+
+extern "C" int stkfmm_plan_execute(stkfmm_plan *p, const stkfmm_array *sc, const stkfmm_array *sv,
+                                   const stkfmm_array *sv2, const stkfmm_array *tc, const stkfmm_out_array *tv,
+                                   const int *perm, void *stream) {
+    if (!p)
+        return set_error(STKFMM_ERR_ARG, "plan is NULL");
+    return plan_enqueue(p, sc, sv, sv2, tc, tv, perm, (hipStream_t)stream, true);
+}
+
+extern "C" int stkfmm_plan_capture_graph(stkfmm_plan *p, const stkfmm_array *sc, const stkfmm_array *sv,
+                                         const stkfmm_array *sv2, const stkfmm_array *tc,
+                                         const stkfmm_out_array *tv, const int *perm, void *stream) {
+    if (!p)
+        return set_error(STKFMM_ERR_ARG, "plan is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    hipStream_t cap = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "capture stream");
+    if (p->gexec) {
+        (void)hipGraphExecDestroy(p->gexec);
+        p->gexec = nullptr;
+    }
+    if (p->graph) {
+        (void)hipGraphDestroy(p->graph);
+        p->graph = nullptr;
+    }
+    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    int rc = STKFMM_OK;
+    if (e == hipSuccess) {
+        rc = plan_enqueue(p, sc, sv, sv2, tc, tv, perm, cap, false);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(cap, &g);
+        if (rc == STKFMM_OK && e2 != hipSuccess)
+            rc = hip_error(e2, "end capture");
+        if (rc == STKFMM_OK) {
+            p->graph = g;
+            e2 = hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0);
+            if (e2 != hipSuccess)
+                rc = hip_error(e2, "graph instantiate");
+        } else if (g) {
+            (void)hipGraphDestroy(g);
+        }
+    } else {
+        rc = hip_error(e, "begin capture");
+    }
+    (void)hipStreamDestroy(cap);
+    (void)s;
+    return rc;
+}
+
+extern "C" int stkfmm_plan_launch_graph(stkfmm_plan *p, void *stream) {
+    if (!p)
+        return set_error(STKFMM_ERR_ARG, "plan is NULL");
+    if (!p->gexec)
+        return set_error(STKFMM_ERR_ARG, "no graph captured for this plan");
+    HIP_TRY(hipGraphLaunch(p->gexec, (hipStream_t)stream), "graph launch");
+    return STKFMM_OK;
+}
+
+extern "C" int stkfmm_plan_stats(const stkfmm_plan *p, long long *pairs, int *n_items, int *n_tiles, int *grid) {
+    if (!p)
+        return set_error(STKFMM_ERR_ARG, "plan is NULL");
+    if (pairs)
+        *pairs = p->pairs;
+    if (n_items)
+        *n_items = p->n_items;
+    if (n_tiles)
+        *n_tiles = p->n_tiles;
+    if (grid)
+        *grid = p->grid;
+    return STKFMM_OK;
+}
+
+extern "C" int stkfmm_plan_set_timing(stkfmm_plan *p, int enable) {
+    if (!p)
+        return set_error(STKFMM_ERR_ARG, "plan is NULL");
+    if (enable && p->ev.empty()) {
+        p->ev.resize(3 * stkfmm_plan::kRing, nullptr);
+        for (auto &e : p->ev)
+            HIP_TRY(hipEventCreate(&e), "event create");
+    }
+    p->timing = enable != 0;
+    p->n_timed = 0;
+    return STKFMM_OK;
+}
+
+extern "C" int stkfmm_plan_kernel_ms(const stkfmm_plan *p, float *item_ms, float *reduce_ms, int *count) {
+    if (!p)
+        return set_error(STKFMM_ERR_ARG, "plan is NULL");
+    const int n = (int)std::min<long long>(p->n_timed, stkfmm_plan::kRing);
+    double sa = 0.0, sb = 0.0;
+    int ok = 0;
+    for (int k = 0; k < n; ++k) {
+        float a = 0.f, b = 0.f;
+        if (hipEventElapsedTime(&a, p->ev[3 * k], p->ev[3 * k + 1]) != hipSuccess ||
+            hipEventElapsedTime(&b, p->ev[3 * k + 1], p->ev[3 * k + 2]) != hipSuccess)
+            return set_error(STKFMM_ERR_HIP, "timing events not complete (synchronise the stream first)");
+        sa += a;
+        sb += b;
+        ++ok;
+    }
+    if (item_ms)
+        *item_ms = ok ? (float)(sa / ok) : -1.f;
+    if (reduce_ms)
+        *reduce_ms = ok ? (float)(sb / ok) : -1.f;
+    if (count)
+        *count = ok;
+    return STKFMM_OK;
+}
+
+extern "C" int stkfmm_plan_destroy(stkfmm_plan *p) {
+    plan_free(p);
+    return STKFMM_OK;
+}
+
+// ---- direct sums (boundary 1) and Ker_t entries (boundary 2) --------------------------------------
+
+namespace {
+std::mutex g_mu;
+
+// cache of single-group plans for direct sums, keyed by shape
+struct DirectKey {
+    int device, uk, prec, n_src, n_trg;
+    bool operator==(const DirectKey &o) const {
+        return device == o.device && uk == o.uk && prec == o.prec && n_src == o.n_src && n_trg == o.n_trg;
+    }
+};
+std::list<std::pair<DirectKey, stkfmm_plan *>> g_direct_plans;
+constexpr size_t kMaxDirectPlans = 16;
+
+int direct_plan(int uk, int prec, int n_src, int n_trg, stkfmm_plan **out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    DirectKey key{dev, uk, prec, n_src, n_trg};
+    for (auto it = g_direct_plans.begin(); it != g_direct_plans.end(); ++it) {
+        if (it->first == key) {
+            g_direct_plans.splice(g_direct_plans.begin(), g_direct_plans, it);
+            *out = it->second;
+            return STKFMM_OK;
+        }
+    }
+    const int gb = 0, ge = n_trg, gp[2] = {0, 1}, rb = 0, re = n_src;
+    stkfmm_plan *p = nullptr;
+    int rc = stkfmm_plan_create(&p, uk, prec, n_src, n_trg, 1, &gb, &ge, nullptr, gp, &rb, &re, nullptr);
+    if (rc)
+        return rc;
+    g_direct_plans.emplace_front(key, p);
+    while (g_direct_plans.size() > kMaxDirectPlans) {
+        plan_free(g_direct_plans.back().second);
+        g_direct_plans.pop_back();
+    }
+    *out = p;
+    return STKFMM_OK;
+}
+
+// grow-only device buffers for the host-pointer entries
+struct Workspace {
+    int device = -1;
+    // slots 0-3: host-pointer direct sums; slot 4: adjoint partial sums
+    double *buf[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    size_t cap[5] = {0, 0, 0, 0, 0};
+};
+std::vector<Workspace> g_ws;
+
+int ws_get(int slot, size_t n, double **out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    Workspace *w = nullptr;
+    for (auto &x : g_ws)
+        if (x.device == dev)
+            w = &x;
+    if (!w) {
+        g_ws.emplace_back();
+        w = &g_ws.back();
+        w->device = dev;
+    }
+    if (w->cap[slot] < n) {
+        (void)hipFree(w->buf[slot]);
+        w->buf[slot] = nullptr;
+        w->cap[slot] = 0;
+        HIP_TRY(hipMalloc((void **)&w->buf[slot], sizeof(double) * std::max<size_t>(n, 1)), "workspace");
+        w->cap[slot] = n;
+    }
+    *out = w->buf[slot];
+    return STKFMM_OK;
+}
+
+int eval_device_uk(int uk, int prec, int n_src, const double *sc, const double *sv, int n_trg, const double *tc,
+                   double *tv, hipStream_t s) {
+    if (n_src < 0 || n_trg < 0)
+        return set_error(STKFMM_ERR_ARG, "negative point count");
+    if (n_src == 0 || n_trg == 0)
+        return STKFMM_OK;
+    if (!sc || !sv || !tc || !tv)
+        return set_error(STKFMM_ERR_ARG, "NULL buffer");
+    stkfmm_plan *p = nullptr;
+    int rc = direct_plan(uk, prec, n_src, n_trg, &p);
+    if (rc)
+        return rc;
+    const int ks = uker_ks(uk), ks1 = uker_ks1(uk), kt = uker_kt(uk);
+    stkfmm_array a_sc{sc, 3, 1}, a_sv{sv, ks, 1}, a_sv2{sv + ks1, ks, 1}, a_tc{tc, 3, 1};
+    stkfmm_out_array a_tv{tv, kt, 1};
+    return plan_enqueue(p, &a_sc, &a_sv, &a_sv2, &a_tc, &a_tv, nullptr, s, false);
+}
+
+int eval_host_uk(int uk, int prec, int n_src, const double *sc, const double *sv, int n_trg, const double *tc,
+                 double *tv) {
+    if (n_src < 0 || n_trg < 0)
+        return set_error(STKFMM_ERR_ARG, "negative point count");
+    if (n_src == 0 || n_trg == 0)
+        return STKFMM_OK;
+    if (!sc || !sv || !tc || !tv)
+        return set_error(STKFMM_ERR_ARG, "NULL buffer");
+    const size_t ks = uker_ks(uk), kt = uker_kt(uk);
+    double *d_sc, *d_sv, *d_tc, *d_tv;
+    int rc;
+    if ((rc = ws_get(0, 3 * (size_t)n_src, &d_sc)) || (rc = ws_get(1, ks * n_src, &d_sv)) ||
+        (rc = ws_get(2, 3 * (size_t)n_trg, &d_tc)) || (rc = ws_get(3, kt * n_trg, &d_tv)))
+        return rc;
+    HIP_TRY(hipMemcpy(d_sc, sc, sizeof(double) * 3 * n_src, hipMemcpyHostToDevice), "copy srcCoord");
+    HIP_TRY(hipMemcpy(d_sv, sv, sizeof(double) * ks * n_src, hipMemcpyHostToDevice), "copy srcValue");
+    HIP_TRY(hipMemcpy(d_tc, tc, sizeof(double) * 3 * n_trg, hipMemcpyHostToDevice), "copy trgCoord");
+    HIP_TRY(hipMemcpy(d_tv, tv, sizeof(double) * kt * n_trg, hipMemcpyHostToDevice), "copy trgValue");
+    rc = eval_device_uk(uk, prec, n_src, d_sc, d_sv, n_trg, d_tc, d_tv, nullptr);
+    if (rc)
+        return rc;
+    HIP_TRY(hipMemcpy(tv, d_tv, sizeof(double) * kt * n_trg, hipMemcpyDeviceToHost), "copy trgValue back");
+    return STKFMM_OK;
+}
+
+int resolve(unsigned kernel, unsigned p2p, int precision, int *uk) {
+    if (precision != STKFMM_PREC_FP64 && precision != STKFMM_PREC_MIXED)
+        return set_error(STKFMM_ERR_ARG, "unknown precision %d", precision);
+    return stkfmm_get_uker(kernel, p2p, uk);
+}
+} // namespace
+
+extern "C" int stkfmm_p2p_eval(unsigned kernel, unsigned p2p, int precision, int nSrc, const double *srcCoord,
+                               const double *srcValue, int nTrg, const double *trgCoord, double *trgValue) {
+    int uk;
+    int rc = resolve(kernel, p2p, precision, &uk);
+    if (rc)
+        return rc;
+    std::lock_guard<std::mutex> lk(g_mu);
+    return eval_host_uk(uk, precision, nSrc, srcCoord, srcValue, nTrg, trgCoord, trgValue);
+}
+
+extern "C" int stkfmm_evaluate_kernel(unsigned kernel, int nThreads, unsigned p2p, int nSrc, const double *srcCoord,
+                                      const double *srcValue, int nTrg, const double *trgCoord, double *trgValue) {
+    (void)nThreads;
+    return stkfmm_p2p_eval(kernel, p2p, STKFMM_PREC_FP64, nSrc, srcCoord, srcValue, nTrg, trgCoord, trgValue);
+}
+
+extern "C" int stkfmm_p2p_eval_device(unsigned kernel, unsigned p2p, int precision, int nSrc,
+                                      const double *d_srcCoord, const double *d_srcValue, int nTrg,
+                                      const double *d_trgCoord, double *d_trgValue, void *stream) {
+    int uk;
+    int rc = resolve(kernel, p2p, precision, &uk);
+    if (rc)
+        return rc;
+    std::lock_guard<std::mutex> lk(g_mu);
+    return eval_device_uk(uk, precision, nSrc, d_srcCoord, d_srcValue, nTrg, d_trgCoord, d_trgValue,
+                          (hipStream_t)stream);
+}
+
+extern "C" int stkfmm_ker_eval(int uker, const double *r_src, int src_cnt, const double *v_src, int dof,
+                               const double *r_trg, int trg_cnt, double *v_trg) {
+    if (uker < 0 || uker >= STKFMM_UKER_COUNT)
+        return set_error(STKFMM_ERR_ARG, "unknown micro-kernel %d", uker);
+    if (dof != 1)
+        return set_error(STKFMM_ERR_UNSUPPORTED, "dof=%d: only dof=1 (as passed by FMMData.cpp:280) is supported",
+                         dof);
+    std::lock_guard<std::mutex> lk(g_mu);
+    return eval_host_uk(uker, STKFMM_PREC_FP64, src_cnt, r_src, v_src, trg_cnt, r_trg, v_trg);
+}
+
+#define STKFMM_KER_T(NAME, UK)                                                                     \
+    extern "C" void stkfmm_ker_##NAME(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg,  \
+                                      int trg_cnt, double *v_trg, void *mem_mgr) {                 \
+        (void)mem_mgr;                                                                             \
+        if (stkfmm_ker_eval(UK, r_src, src_cnt, v_src, dof, r_trg, trg_cnt, v_trg) != STKFMM_OK)  \
+            fprintf(stderr, "stkfmm_ker_" #NAME ": %s\n", g_last_error.c_str());                   \
+    }
+
+STKFMM_KER_T(laplace_p, STKFMM_UKER_LAPLACE_P)
+STKFMM_KER_T(laplace_dipolep, STKFMM_UKER_LAPLACE_DIPOLEP)
+STKFMM_KER_T(laplace_pgrad, STKFMM_UKER_LAPLACE_PGRAD)
+STKFMM_KER_T(laplace_dipolepgrad, STKFMM_UKER_LAPLACE_DIPOLEPGRAD)
+STKFMM_KER_T(laplace_pgradgrad, STKFMM_UKER_LAPLACE_PGRADGRAD)
+STKFMM_KER_T(laplace_dipolepgradgrad, STKFMM_UKER_LAPLACE_DIPOLEPGRADGRAD)
+STKFMM_KER_T(stokes_vel, STKFMM_UKER_STOKES_VEL)
+STKFMM_KER_T(stokes_pvel, STKFMM_UKER_STOKES_PVEL)
+STKFMM_KER_T(stokes_doublepvel, STKFMM_UKER_STOKES_DOUBLEPVEL)
+STKFMM_KER_T(stokes_regvel, STKFMM_UKER_STOKES_REGVEL)
+STKFMM_KER_T(rpy_u, STKFMM_UKER_RPY_U)
+STKFMM_KER_T(rpy_ulapu, STKFMM_UKER_RPY_ULAPU)
+STKFMM_KER_T(stk_ulapu, STKFMM_UKER_STK_ULAPU)
+
+// ---- adjoint ------------------------------------------------------------------------------------------
+
+extern "C" int stkfmm_p2p_vjp_device(int uker, int precision, int nSrc, const double *d_srcCoord,
+                                     const double *d_srcValue, int nTrg, const double *d_trgCoord,
+                                     const double *d_w, double *d_g_src_value, double *d_g_src_coord,
+                                     double *d_g_trg_coord, void *stream) {
+    if (uker < 0 || uker >= STKFMM_UKER_COUNT || !uker_has_vjp(uker))
+        return set_error(STKFMM_ERR_UNSUPPORTED, "no adjoint for micro-kernel %d", uker);
+    if (precision != STKFMM_PREC_FP64 && precision != STKFMM_PREC_MIXED)
+        return set_error(STKFMM_ERR_ARG, "unknown precision %d", precision);
+    if (nSrc < 0 || nTrg < 0)
+        return set_error(STKFMM_ERR_ARG, "negative point count");
+    if ((nSrc > 0 && (!d_srcCoord || !d_srcValue)) || (nTrg > 0 && (!d_trgCoord || !d_w)))
+        return set_error(STKFMM_ERR_ARG, "NULL buffer");
+    int n_cu = 0;
+    int rc = device_cu_count(&n_cu);
+    if (rc)
+        return rc;
+    const int ks = uker_ks(uker);
+    // enough workgroups to fill the device: about 8 per CU, chunks of at least 256 entities
+    auto chunks = [&](int n_lane, int n_other) {
+        const long long lanes_wg = std::max(1, (n_lane + kWave - 1) / kWave);
+        long long c = (8LL * n_cu + lanes_wg - 1) / lanes_wg;
+        c = std::min<long long>(c, std::max(1, n_other / 256));
+        return (int)std::max(1LL, c);
+    };
+    const int nct = chunks(nTrg, nSrc), ncs = chunks(nSrc, nTrg);
+    std::lock_guard<std::mutex> lk(g_mu);
+    double *partial = nullptr;
+    const size_t need = std::max((size_t)nct * 3 * nTrg, (size_t)ncs * (3 + ks) * nSrc);
+    if ((rc = ws_get(4, need, &partial)))
+        return rc;
+    VjpArgs a{d_srcCoord, d_srcValue, d_trgCoord, d_w, nSrc, nTrg, uker_scale(uker),
+              d_g_src_value, d_g_src_coord, d_g_trg_coord};
+    HIP_TRY(launch_vjp(uker, precision, a, partial, nct, ncs, (hipStream_t)stream), "vjp launch");
+    return STKFMM_OK;
+}

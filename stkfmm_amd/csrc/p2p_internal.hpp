@@ -1,0 +1,104 @@
+// p2p_internal.hpp -- types shared by the HIP kernels (p2p_kernels.hip) and the host code
+// (p2p_capi.cpp) of libstkfmm_p2p.so.  Not part of the public C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace stkp2p {
+
+// Target-per-lane tiling: one workgroup = one wavefront of 64 lanes; each lane owns T targets,
+// so a work item covers a tile of 64*T consecutive targets.  Sources are staged kSrcTile at a time
+// through LDS.
+constexpr int kWave = 64;
+constexpr int kSrcTile = 64;
+// mixed mode: FP32 partial sums over this many sources before they are added to the FP64
+// accumulator of the target
+constexpr int kMixedAccBlock = 16;
+
+// one work item: a target tile against a list of source ranges
+struct Item {
+    int trg_begin;  // first target (index into the target arrays)
+    int trg_count;  // targets in the tile, 1..64*T
+    int rng_begin;  // source ranges [rng_begin, rng_end) of the plan's range array
+    int rng_end;
+    int slot;       // partial-sum slot this item writes
+    int level;      // octree level of the target group (mixed-mode scaling by 2^level)
+    int cost;       // sources in the item (scheduling key)
+    int pad;
+};
+
+// a target tile and the partial-sum slots that cover it
+struct Tile {
+    int trg_begin;
+    int trg_count;
+    int slot_begin;
+    int slot_end;
+};
+
+// element (i, k) of a strided device array is p[i*s + k*fs]
+struct DArr {
+    const double *p;
+    long long s;
+    long long fs;
+};
+struct OArr {
+    double *p;
+    long long s;
+    long long fs;
+};
+
+struct ItemArgs {
+    DArr sc;   // source coordinates
+    DArr v1;   // source values [0, KS1)
+    DArr v2;   // source values [KS1, KS) (fused single+double layer only)
+    DArr tc;   // target coordinates
+    const Item *items;
+    const int2 *ranges;   // (src_begin, src_end)
+    int n_items;
+    int *counter;         // persistent mode: next item to take; null: one workgroup per item
+    double *partial;      // [slot][KT][64*T]
+};
+
+struct ReduceArgs {
+    const Tile *tiles;
+    int n_tiles;
+    int kt;
+    int tt;               // 64*T
+    const double *partial;
+    OArr out;
+    const int *perm;      // output row of sorted target i (null: identity)
+    double scale;         // ScaleFactor of the kernel
+};
+
+// adjoint of a direct sum (target-centric and source-centric passes)
+struct VjpArgs {
+    const double *sc;  // AoS source coords [n_src*3]
+    const double *sv;  // AoS source values [n_src*KS]
+    const double *tc;  // AoS target coords [n_trg*3]
+    const double *w;   // AoS cotangent [n_trg*KT]
+    int n_src;
+    int n_trg;
+    double scale;
+    double *g_sv;      // [n_src*KS] (may be null)
+    double *g_sc;      // [n_src*3]  (may be null)
+    double *g_tc;      // [n_trg*3]  (may be null)
+};
+
+// kernel-table queries (host)
+int uker_ks(int uk);
+int uker_ks1(int uk);
+int uker_kt(int uk);
+double uker_scale(int uk);
+bool uker_has_vjp(int uk);
+
+// launchers (host); return hipSuccess or the launch error
+hipError_t launch_items(int uk, int prec, int T, int grid, const ItemArgs &a, hipStream_t s);
+hipError_t launch_reduce(const ReduceArgs &a, hipStream_t s);
+hipError_t launch_fill_int(int *p, int value, hipStream_t s);
+hipError_t items_occupancy(int uk, int prec, int T, int *blocks_per_cu);
+// chunk counts: the target pass splits the sources into nchunk_t chunks, the source pass splits the
+// targets into nchunk_s chunks; partial must hold max(nchunk_t*3*n_trg, nchunk_s*(3+KS)*n_src)
+hipError_t launch_vjp(int uk, int prec, const VjpArgs &a, double *partial, int nchunk_t, int nchunk_s,
+                      hipStream_t s);
+
+} // namespace stkp2p

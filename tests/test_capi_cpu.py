@@ -1,0 +1,120 @@
+"""C ABI of libstkfmm_p2p.so on the CPU: loads, exports every declared symbol, kernel table and host
+argument checks.  No compute call is made (there is no GPU here)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import stkfmm_amd as S
+from stkfmm_amd import _capi
+from tests.conftest import ROOT
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "stkfmm_p2p.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(stkfmm_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_function():
+    L = S.lib()
+    names = header_functions()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert sorted(names) == sorted(_capi.EXPORTS)
+
+
+# getKernelDimension (Lib/src/STKFMM.cpp:33-40): ker_dim[0], surf_dim, ker_dim[1] of the kernels built in
+# STKFMM.cpp:18-31; surf_dim of kernels without a double layer is PVFMM's default (3 + ker_dim[0]).
+DIMS = {1: (1, 3, 4), 2: (1, 3, 10), 4: (9, 12, 10), 8: (3, 6, 3), 16: (4, 7, 6), 32: (4, 7, 3), 64: (7, 10, 6),
+        128: (4, 9, 4), 256: (4, 9, 16), 512: (4, 9, 7), 1024: (4, 9, 9)}
+NAMES = {1: "laplace_PGrad", 2: "laplace_PGradGrad", 4: "laplace_QPGradGrad", 8: "stokes_vel", 16: "rpy_ulapu",
+         32: "stokes_regvel", 64: "stokes_regftvelomega", 128: "stokes_PVel", 256: "stokes_PVelGrad",
+         512: "stokes_PVelLaplacian", 1024: "stokes_Traction"}
+
+
+@pytest.mark.parametrize("k", sorted(DIMS))
+def test_kernel_dimensions_and_names(k):
+    assert tuple(S.get_kernel_dimension(k)) == DIMS[k]
+    assert S.get_kernel_name(k) == NAMES[k]
+
+
+# FMMData::evaluateKernel sub-kernel selection (Lib/src/FMMData.cpp:261-268) with the BuildKernel lists
+UKERS = {(1, 1): 2, (1, 2): 3, (1, 4): 2, (2, 1): 4, (2, 2): 5, (2, 4): 4, (8, 1): 6, (8, 4): 6, (16, 1): 11,
+         (16, 4): 12, (32, 1): 9, (32, 4): 6, (128, 1): 7, (128, 2): 8, (128, 4): 7}
+
+
+@pytest.mark.parametrize("kp", sorted(UKERS))
+def test_sub_kernel_selection(kp):
+    assert int(S.get_uker(*kp)) == UKERS[kp]
+
+
+@pytest.mark.parametrize("kp", [(8, 2), (16, 2), (32, 2)])
+def test_missing_double_layer(kp):
+    with pytest.raises(S.PPKernelNotFound, match="not found for direct evaluation"):
+        S.get_uker(*kp)
+
+
+def test_unknown_kernel_and_ppkernel():
+    with pytest.raises(S.KernelNotFound):
+        S.get_kernel_dimension(3)
+    with pytest.raises(S.KernelNotFound):
+        S.get_uker(2048, 1)
+    with pytest.raises(S.PPKernelNotFound):
+        S.get_uker(1, 3)
+
+
+@pytest.mark.parametrize("k", [4, 64, 256, 512, 1024])
+def test_unimplemented_kernels_raise(k):
+    with pytest.raises(S.Unsupported):
+        S.get_uker(k, 1)
+
+
+def test_uker_dims():
+    expect = [(1, 1), (3, 1), (1, 4), (3, 4), (1, 10), (3, 10), (3, 3), (4, 4), (9, 4), (4, 3), (4, 3), (4, 6),
+              (3, 6), (13, 4)]
+    assert [S.uker_dims(u) for u in range(14)] == expect
+    with pytest.raises(S.ArgumentError):
+        S.uker_dims(14)
+
+
+def test_empty_inputs_are_a_no_op():
+    # nSrc = 0 or nTrg = 0 returns before touching the device
+    trg = np.random.default_rng(0).random((5, 3))
+    val = np.ones((5, 4))
+    S.evaluate_kernel(S.KERNEL.PVel, 0, S.PPKERNEL.SLS2T, np.zeros((0, 3)), np.zeros((0, 4)), trg, val)
+    assert np.all(val == 1.0)
+    S.evaluate_kernel(S.KERNEL.PVel, 0, S.PPKERNEL.SLS2T, trg, val, np.zeros((0, 3)), np.zeros((0, 4)))
+
+
+def test_host_argument_checks():
+    rng = np.random.default_rng(1)
+    src, trg = rng.random((10, 3)), rng.random((6, 3))
+    with pytest.raises(S.ArgumentError):  # wrong value dimension
+        S.evaluate_kernel(S.KERNEL.PVel, 0, S.PPKERNEL.SLS2T, src, np.zeros((10, 3)), trg, np.zeros((6, 4)))
+    with pytest.raises(S.ArgumentError):  # float32
+        S.evaluate_kernel(S.KERNEL.PVel, 0, S.PPKERNEL.SLS2T, src.astype(np.float32), np.zeros((10, 4)), trg,
+                          np.zeros((6, 4)))
+    with pytest.raises(S.ArgumentError):  # non-contiguous
+        S.evaluate_kernel(S.KERNEL.LapPGrad, 0, S.PPKERNEL.SLS2T, src, np.zeros((10, 2))[:, :1], trg,
+                          np.zeros((6, 4)))
+    with pytest.raises(S.PPKernelNotFound):
+        S.evaluate_kernel(S.KERNEL.RPY, 0, S.PPKERNEL.DLS2T, src, np.zeros((10, 4)), trg, np.zeros((6, 6)))
+
+
+def test_stk3dfmm_inactive_kernel():
+    fmm = S.Stk3DFMM(10, 2000, S.PAXIS.NONE, S.KERNEL.PVel | S.KERNEL.LapPGrad)
+    assert fmm.is_kernel_active(S.KERNEL.PVel) and not fmm.is_kernel_active(S.KERNEL.RPY)
+    rng = np.random.default_rng(2)
+    with pytest.raises(S.KernelNotActive):
+        fmm.evaluate_kernel(S.KERNEL.RPY, 0, S.PPKERNEL.SLS2T, rng.random((4, 3)), np.zeros((4, 4)),
+                            rng.random((3, 3)), np.zeros((3, 6)))
+
+
+def test_ker_t_dof_must_be_one():
+    rng = np.random.default_rng(3)
+    with pytest.raises(S.Unsupported):
+        S.ker_eval(S.UKER.STOKES_PVEL, rng.random((4, 3)), np.zeros((4, 4)), rng.random((3, 3)),
+                   np.zeros((3, 4)), dof=2)

@@ -1,0 +1,111 @@
+"""GPU parity: adjoint kernels (stkfmm_p2p_vjp_device) and the torch.autograd.Function."""
+import numpy as np
+import pytest
+
+import stkfmm_amd as S
+from oracle import oracle as O
+from stkfmm_amd import workloads as W
+from tests.conftest import source_values
+from tests.gpu_util import host, say, to_dev, zeros
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("uk", [7, 8, 11, 13])
+def test_vjp_vs_oracle(uk, prec):
+    say(f"vjp uker={uk} prec={prec}")
+    rng = np.random.default_rng(500 + uk)
+    ns, nt = 300, 210
+    src = rng.random((ns, 3))
+    trg = np.concatenate([src[:9], rng.random((nt - 9, 3))])
+    v = source_values(uk, ns, rng)
+    w = rng.normal(size=(nt, O.trg_dim(uk)))
+    ref = O.direct_vjp(uk, src, v, trg, w)
+    got = S.vjp_device(uk, to_dev(src), to_dev(v), to_dev(trg), to_dev(w), precision=prec)
+    tol = 1e-11 if prec == 0 else 1e-5
+    for g, r, name in zip(got, ref, ("g_src_value", "g_src_coord", "g_trg_coord")):
+        l2, mx = O.max_error(host(g), r)
+        assert l2 < tol and mx < tol, (name, l2, mx)
+
+
+def test_autograd_function_matches_vjp():
+    import torch
+    from stkfmm_amd.autograd import p2p
+    say("autograd")
+    rng = np.random.default_rng(600)
+    src = to_dev(rng.random((120, 3))).requires_grad_(True)
+    val = to_dev(source_values(13, 120, rng)).requires_grad_(True)
+    trg = to_dev(rng.random((90, 3))).requires_grad_(True)
+    w = to_dev(rng.normal(size=(90, 4)))
+    u = p2p(S.UKER.STOKES_PVEL_FUSED, src, val, trg)
+    (u * w).sum().backward()
+    ref = O.direct_vjp(13, host(src), host(val), host(trg), host(w))
+    for g, r in zip((val.grad, src.grad, trg.grad), ref):
+        assert O.max_error(host(g), r)[0] < 1e-11
+    # torch's own finite-difference check on a tiny problem
+    s2 = to_dev(rng.random((4, 3))).requires_grad_(True)
+    v2 = to_dev(source_values(11, 4, rng)).requires_grad_(True)
+    t2 = to_dev(rng.random((3, 3))).requires_grad_(True)
+    assert torch.autograd.gradcheck(lambda a, b, c: p2p(S.UKER.RPY_ULAPU, a, b, c), (s2, v2, t2), eps=1e-6,
+                                    atol=1e-6, rtol=1e-5)
+
+
+@pytest.mark.slow
+def test_workload4_adjoint_full_size():
+    """65,536 x 65,536 PVel: gradients vs central differences (h = 1e-6) on 64 sampled coordinates and
+    vs the oracle gradient on sampled entries."""
+    import torch
+    from stkfmm_amd.autograd import p2p
+    say("workload 4 full size")
+    src, val, trg, w = W.adjoint_pvel()
+    d_src = to_dev(src).requires_grad_(True)
+    d_val = to_dev(val).requires_grad_(True)
+    d_trg = to_dev(trg).requires_grad_(True)
+    d_w = to_dev(w)
+    u = p2p(S.UKER.STOKES_PVEL, d_src, d_val, d_trg)
+    (u * d_w).sum().backward()
+    g_src, g_val, g_trg = host(d_src.grad), host(d_val.grad), host(d_trg.grad)
+    rng = np.random.default_rng(4)
+    h = 1e-6
+
+    def field(s, v, t):
+        out = zeros(t.shape[0], 4)
+        S.uker_device(S.UKER.STOKES_PVEL, s, v, t, out)
+        return out
+
+    ds, dv, dt = to_dev(src), to_dev(val), to_dev(trg)
+    # target coordinates: only u_t changes
+    for _ in range(22):
+        t_i, j = int(rng.integers(len(trg))), int(rng.integers(3))
+        tp, tm = trg[t_i:t_i + 1].copy(), trg[t_i:t_i + 1].copy()
+        tp[0, j] += h
+        tm[0, j] -= h
+        fd = float(((field(ds, dv, to_dev(tp)) - field(ds, dv, to_dev(tm))) * d_w[t_i:t_i + 1]).sum()) / (2 * h)
+        assert abs(fd - g_trg[t_i, j]) <= 1e-6 * abs(g_trg[t_i, j]) + 1e-6, ("trg", t_i, j, fd, g_trg[t_i, j])
+    # source coordinates and forces: every target changes
+    for kind in ("src", "val"):
+        for _ in range(21):
+            s_i, j = int(rng.integers(len(src))), int(rng.integers(3))
+            arr = src if kind == "src" else val
+            p, m = arr.copy(), arr.copy()
+            p[s_i, j] += h
+            m[s_i, j] -= h
+            if kind == "src":
+                lp = field(to_dev(p), dv, dt)
+                lm = field(to_dev(m), dv, dt)
+                g = g_src[s_i, j]
+            else:
+                lp = field(ds, to_dev(p), dt)
+                lm = field(ds, to_dev(m), dt)
+                g = g_val[s_i, j]
+            fd = float(((lp - lm) * d_w).sum()) / (2 * h)
+            assert abs(fd - g) <= 1e-5 * abs(g) + 1e-5, (kind, s_i, j, fd, g)
+    # oracle gradients on sampled sources / targets
+    si = rng.choice(len(src), 48, replace=False)
+    ti = rng.choice(len(trg), 48, replace=False)
+    gv_ref, gs_ref, _ = O.direct_vjp(7, src[si], val[si], trg, w)
+    _, _, gt_ref = O.direct_vjp(7, src, val, trg[ti], w[ti])
+    assert O.max_error(g_val[si], gv_ref)[0] < 1e-11
+    assert O.max_error(g_src[si], gs_ref)[0] < 1e-11
+    assert O.max_error(g_trg[ti], gt_ref)[0] < 1e-11

@@ -1,0 +1,223 @@
+"""GPU parity: batched work-list plans (U-list of an adaptive octree, many independent direct sums,
+HIP graph replay) against the oracle's range-list driver."""
+import numpy as np
+import pytest
+
+import stkfmm_amd as S
+from oracle import oracle as O
+from stkfmm_amd import workloads as W
+from tests.gpu_util import FP64_TOL, MIXED_TOL, host, say, to_dev, zeros
+
+pytestmark = pytest.mark.gpu
+
+
+def sorted_problem(x, cap, uk, seed):
+    tr = S.Tree(x, cap)
+    xs = x[tr.perm]
+    rng = np.random.default_rng(seed)
+    if uk == S.UKER.STOKES_PVEL_FUSED:
+        sl = rng.normal(size=(len(x), 4))
+        dl = rng.normal(size=(len(x), 9))
+        return tr, xs, sl, dl
+    ks = O.src_dim(uk)
+    v = rng.normal(size=(len(x), ks))
+    if uk in (S.UKER.RPY_ULAPU, S.UKER.STOKES_REGVEL):
+        v[:, 3] = rng.uniform(1e-4, 5e-4, len(x))
+    return tr, xs, v, None
+
+
+def run_plan(tr, uk, prec, xs, v, dl=None, tune=None, perm=None, out_rows=None):
+    plan = S.Plan.from_tree(tr, uk, prec, tune=tune)
+    kt = O.trg_dim(uk)
+    out = zeros(out_rows or tr.n, kt)
+    d_x = to_dev(xs)
+    plan.execute(d_x, to_dev(v), d_x, out, src_value2=None if dl is None else to_dev(dl),
+                 perm=None if perm is None else to_dev(perm))
+    return host(out), plan
+
+
+def oracle_leaves(tr, uk, xs, v, dl, sel):
+    vals = v if dl is None else np.concatenate([v, dl], axis=1)
+    return O.ranges(uk, xs, vals, xs, tr.leaf_begin, tr.leaf_end, tr.ulist_ptr, tr.rng_begin, tr.rng_end, sel=sel)
+
+
+def leaf_rows(tr, sel):
+    return np.concatenate([np.arange(tr.leaf_begin[l], tr.leaf_end[l]) for l in sel])
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("uk", [13, 2, 11, 6, 9, 7, 8, 4])
+def test_small_tree_all_leaves(uk, prec):
+    say(f"small tree uker={uk} prec={prec}")
+    g = np.random.default_rng(uk)
+    x = W.wrap(np.concatenate([g.random((6000, 3)), 0.6 + 0.02 * g.normal(size=(6000, 3))]))
+    tr, xs, v, dl = sorted_problem(x, 150, uk, 10 + uk)
+    out, plan = run_plan(tr, uk, prec, xs, v, dl)
+    assert plan.stats()["pairs"] == tr.pairs
+    ref = oracle_leaves(tr, uk, xs, v, dl, None)
+    l2, mx = O.max_error(out, ref)
+    tol = FP64_TOL if prec == 0 else MIXED_TOL
+    assert l2 < tol and mx < tol, (l2, mx)
+
+
+@pytest.mark.parametrize("tune", [dict(targets_per_lane=1), dict(targets_per_lane=2, persistent=2),
+                                  dict(targets_per_lane=4, max_src_per_item=300),
+                                  dict(targets_per_lane=2, max_src_per_item=64, waves_per_cu=1)])
+def test_tuning_knobs_do_not_change_results(tune):
+    say(f"tune {tune}")
+    g = np.random.default_rng(7)
+    x = g.random((9000, 3))
+    tr, xs, sl, dl = sorted_problem(x, 200, 13, 77)
+    ref = oracle_leaves(tr, 13, xs, sl, dl, None)
+    out, _ = run_plan(tr, 13, 0, xs, sl, dl, tune=tune)
+    l2, mx = O.max_error(out, ref)
+    assert l2 < FP64_TOL and mx < FP64_TOL
+
+
+def test_permuted_output_and_accumulation():
+    say("perm output")
+    g = np.random.default_rng(8)
+    x = g.random((5000, 3))
+    tr, xs, sl, dl = sorted_problem(x, 100, 13, 88)
+    plan = S.Plan.from_tree(tr, 13, 0)
+    out = to_dev(np.ones((tr.n, 4)))
+    d_x = to_dev(xs)
+    plan.execute(d_x, to_dev(sl), d_x, out, src_value2=to_dev(dl), perm=to_dev(tr.perm))
+    got = host(out) - 1.0
+    ref = oracle_leaves(tr, 13, xs, sl, dl, None)
+    unsorted = np.zeros_like(ref)
+    unsorted[tr.perm] = ref
+    l2, mx = O.max_error(got, unsorted)
+    assert l2 < FP64_TOL and mx < FP64_TOL
+
+
+def test_soa_layouts():
+    say("SoA layouts")
+    g = np.random.default_rng(9)
+    x = g.random((4000, 3))
+    tr, xs, v, _ = sorted_problem(x, 100, 11, 99)
+    plan = S.Plan.from_tree(tr, 11, 0)
+    xs_soa = to_dev(xs.T.copy())          # [3, n]
+    v_soa = to_dev(v.T.copy())            # [4, n]
+    out_soa = zeros(6, tr.n)
+    plan.execute(xs_soa.t(), v_soa.t(), xs_soa.t(), out_soa.t())
+    ref = oracle_leaves(tr, 11, xs, v, None, None)
+    l2, mx = O.max_error(host(out_soa).T, ref)
+    assert l2 < FP64_TOL and mx < FP64_TOL
+
+
+@pytest.mark.slow
+def test_workload2_pvel_ulist_full_size():
+    """1M points, leaf cap 1000: sampled leaves vs the oracle (FP64 and mixed); FP64 linearity."""
+    say("workload 2 tree")
+    x = W.pvel_points()
+    tr = S.Tree(x, 1000)
+    xs = x[tr.perm]
+    sl, dl = W.pvel_values(tr.n, trd=True)
+    d_x, d_sl, d_dl = to_dev(xs), to_dev(sl), to_dev(dl)
+    outs = {}
+    for prec in (0, 1):
+        say(f"workload 2 prec={prec}")
+        plan = S.Plan.from_tree(tr, 13, prec)
+        out = zeros(tr.n, 4)
+        plan.execute(d_x, d_sl, d_x, out, src_value2=d_dl)
+        outs[prec] = host(out)
+    g = np.random.default_rng(2024)
+    sz = tr.leaf_end - tr.leaf_begin
+    sel = np.unique(np.concatenate([g.choice(tr.n_leaves, 12, replace=False), [np.argmax(sz), np.argmin(sz)]]))
+    ref = oracle_leaves(tr, 13, xs, sl, dl, sel)
+    rows = leaf_rows(tr, sel)
+    l2, mx = O.max_error(outs[0][rows], ref[rows])
+    assert l2 < FP64_TOL and mx < FP64_TOL, (l2, mx)
+    l2, mx = O.max_error(outs[1][rows], ref[rows])
+    assert l2 < MIXED_TOL and mx < MIXED_TOL, (l2, mx)
+    # whole-run mixed vs FP64 agreement
+    l2, mx = O.max_error(outs[1], outs[0])
+    assert l2 < MIXED_TOL, (l2, mx)
+    # linearity: doubling every strength doubles the FP64 result exactly
+    plan = S.Plan.from_tree(tr, 13, 0)
+    out2 = zeros(tr.n, 4)
+    plan.execute(d_x, 2.0 * d_sl, d_x, out2, src_value2=2.0 * d_dl)
+    assert np.array_equal(host(out2), 2.0 * outs[0])
+
+
+@pytest.mark.slow
+def test_workload3_rpy_full_size():
+    """4M clustered points, leaf cap 2000, RPY: sampled leaves vs the oracle, mixed and FP64."""
+    say("workload 3 tree")
+    x = W.rpy_points()
+    tr = S.Tree(x, 2000)
+    xs = x[tr.perm]
+    v = W.rpy_values(tr.n)
+    d_x, d_v = to_dev(xs), to_dev(v)
+    outs = {}
+    for prec in (1, 0):
+        say(f"workload 3 prec={prec}")
+        plan = S.Plan.from_tree(tr, 11, prec)
+        out = zeros(tr.n, 6)
+        plan.execute(d_x, d_v, d_x, out)
+        outs[prec] = host(out)
+    g = np.random.default_rng(3)
+    sel = np.unique(g.choice(tr.n_leaves, 8, replace=False))
+    ref = oracle_leaves(tr, 11, xs, v, None, sel)
+    rows = leaf_rows(tr, sel)
+    l2, mx = O.max_error(outs[0][rows], ref[rows])
+    assert l2 < FP64_TOL and mx < FP64_TOL, (l2, mx)
+    l2, mx = O.max_error(outs[1][rows], ref[rows])
+    assert l2 < MIXED_TOL and mx < MIXED_TOL, (l2, mx)
+
+
+def instance_plan(sizes, offsets, prec, uk=6):
+    n = int(offsets[-1])
+    gb, ge = offsets[:-1], offsets[1:]
+    gp = np.arange(len(sizes) + 1)
+    return S.Plan(uk, prec, n, n, gb, ge, None, gp, gb, ge)
+
+
+def test_instances_and_graph_replay():
+    """Many independent Stokeslet direct sums in one launch; graph replays equal the eager result."""
+    import torch
+    say("instances + graph")
+    sizes, offsets, coord, f = W.stokeslet_instances(n_inst=24, n_lo=100, n_hi=2000)
+    plan = instance_plan(sizes, offsets, 0)
+    d_c, d_f = to_dev(coord), to_dev(f)
+    out = zeros(len(coord), 3)
+    plan.execute(d_c, d_f, d_c, out)
+    eager = host(out)
+    for i in range(len(sizes)):
+        a, b = offsets[i], offsets[i + 1]
+        ref = O.direct(6, coord[a:b], f[a:b], coord[a:b])
+        l2, mx = O.max_error(eager[a:b], ref)
+        assert l2 < FP64_TOL and mx < FP64_TOL
+    stream = torch.cuda.Stream()
+    out_g = zeros(len(coord), 3)
+    torch.cuda.synchronize()
+    plan.capture_graph(d_c, d_f, d_c, out_g, stream=stream)
+    for rep in range(4):
+        out_g.zero_()
+        torch.cuda.synchronize()
+        plan.launch_graph(stream=stream)
+        stream.synchronize()
+        assert np.array_equal(host(out_g), eager), rep
+        tmp = d_f * 1.5  # unrelated eager work between replays
+        del tmp
+
+
+@pytest.mark.slow
+def test_workload5_instances_full_size():
+    say("workload 5 full size")
+    sizes, offsets, coord, f = W.stokeslet_instances()
+    d_c, d_f = to_dev(coord), to_dev(f)
+    outs = {}
+    for prec in (0, 1):
+        plan = instance_plan(sizes, offsets, prec)
+        out = zeros(len(coord), 3)
+        plan.execute(d_c, d_f, d_c, out)
+        outs[prec] = host(out)
+    for i in (0, 7, int(np.argmax(sizes)), int(np.argmin(sizes))):
+        a, b = offsets[i], offsets[i + 1]
+        ref = O.direct(6, coord[a:b], f[a:b], coord[a:b])
+        l2, mx = O.max_error(outs[0][a:b], ref)
+        assert l2 < FP64_TOL and mx < FP64_TOL
+        l2, mx = O.max_error(outs[1][a:b], ref)
+        assert l2 < MIXED_TOL and mx < MIXED_TOL
