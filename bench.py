@@ -87,7 +87,15 @@ def main():
     pairs = tree.pairs
     stream = torch.cuda.Stream()
 
+    from stkfmm_amd.autotune import autotune
+
     def run_variant(prec, tune, steps, warmup):
+        if tune == "autotune":
+            scratch = torch.zeros((tree.n, 4), dtype=torch.float64, device=dev)
+            tune, _ = autotune(tree, S.UKER.STOKES_PVEL_FUSED, prec,
+                               dict(src_coord=d_x, src_value=d_sl, trg_coord=d_x, trg_value=scratch,
+                                    src_value2=d_dl, stream=stream))
+            del scratch
         plan = S.Plan.from_tree(tree, S.UKER.STOKES_PVEL_FUSED, prec, tune=tune)
         res = torch.zeros((tree.n, 4), dtype=torch.float64, device=dev)
         with torch.cuda.stream(stream):
@@ -108,7 +116,7 @@ def main():
         assert st["pairs"] == pairs
         return dict(wall_s=wall, ms_per_step=1e3 * wall / steps, gpair_s=pairs * steps / wall / 1e9,
                     item_ms=item_ms, reduce_ms=reduce_ms, timed_launches=count, n_items=st["n_items"],
-                    n_tiles=st["n_tiles"], grid=st["grid"]), res
+                    n_tiles=st["n_tiles"], grid=st["grid"], tune=tune), res
 
     variants = {}
     results = {}
@@ -117,7 +125,9 @@ def main():
         if name == "per-leaf" and args.no_per_leaf:
             continue
         prec = S.PREC.MIXED if name == "mixed" else S.PREC.FP64
-        tune = dict(persistent=3) if name == "per-leaf" else None
+        # batched variants use the autotuner's choice (cached per workload class); the per-leaf
+        # baseline launches every leaf separately with the library's default tiles
+        tune = dict(persistent=3) if name == "per-leaf" else "autotune"
         steps = args.steps if name != "per-leaf" else max(1, min(args.steps, 3))
         warm = args.warmup if name != "per-leaf" else 1
         variants[name], results[name] = run_variant(prec, tune, steps, warm)
@@ -136,7 +146,8 @@ def main():
         peak = PEAK_FP64
     roofline = {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": None,
-                "kernel": f"p2p_items_kernel<STOKES_PVEL_FUSED,{ 'MIXED' if head == 'mixed' else 'FP64'},T=2>",
+                "kernel": f"p2p_items_kernel<STOKES_PVEL_FUSED,{'MIXED' if head == 'mixed' else 'FP64'},"
+                          f"T={(hv['tune'] or {}).get('targets_per_lane', 2)}>",
                 "kernel_ms": hv["item_ms"], "flops_per_pair": FLOPS_PER_PAIR, "pairs_per_launch": pairs,
                 "note": "VALU-bound; peak uses MI355X published vector FP64 78.6 and packed FP32 157.3 TFLOP/s; "
                         "this build keeps packed FP32 disabled, so FP32 runs at the FP64 VALU rate"}
@@ -160,20 +171,22 @@ def main():
                               tree.rng_begin[tree.ulist_ptr[l]:tree.ulist_ptr[l + 1]]).sum())
                          for l in range(tree.n_leaves)], dtype=np.int64)
         vals = np.concatenate([sl, dl], axis=1)
-        # calibrate the CPU rate on a few leaves, then size the sample to about cpu-seconds of work
-        cal = order[:4]
-        t = time.perf_counter()
-        O.ranges(13, xs, vals, xs, tree.leaf_begin, tree.leaf_end, tree.ulist_ptr, tree.rng_begin, tree.rng_end,
-                 sel=cal, nthreads=threads, simd=True)
-        rate = float((sz[cal] * nsrc[cal]).sum()) / max(time.perf_counter() - t, 1e-6)
-        budget = rate * args.cpu_seconds
-        cum = np.cumsum(sz[order] * nsrc[order])
-        sel = order[: max(8, int(np.searchsorted(cum, budget)) + 1)]
+        # evaluate random leaves in chunks until about cpu-seconds of CPU work has been done
+        ref = np.zeros((tree.n, 4))
+        done = []
+        cpu_s = 0.0
+        pos = 0
+        chunk = 4 * threads
+        while pos < tree.n_leaves and cpu_s < args.cpu_seconds:
+            sel_c = order[pos:pos + chunk]
+            pos += len(sel_c)
+            t = time.perf_counter()
+            ref = O.ranges(13, xs, vals, xs, tree.leaf_begin, tree.leaf_end, tree.ulist_ptr, tree.rng_begin,
+                           tree.rng_end, sel=sel_c, nthreads=threads, simd=True, trg_value=ref)
+            cpu_s += time.perf_counter() - t
+            done.append(sel_c)
+        sel = np.concatenate(done)
         sample_pairs = int((sz[sel] * nsrc[sel]).sum())
-        t = time.perf_counter()
-        ref = O.ranges(13, xs, vals, xs, tree.leaf_begin, tree.leaf_end, tree.ulist_ptr, tree.rng_begin,
-                       tree.rng_end, sel=sel, nthreads=threads, simd=True)
-        cpu_s = time.perf_counter() - t
         rows = np.concatenate([np.arange(tree.leaf_begin[l], tree.leaf_end[l]) for l in sel])
         cpu = {"value": sample_pairs / cpu_s / 1e9, "unit": "Gpair/s", "cores": threads, "kind": "port",
                "sample": f"{len(sel)} of {tree.n_leaves} leaves (random), {rows.size} targets, "

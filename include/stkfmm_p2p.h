@@ -178,6 +178,8 @@ typedef struct stkfmm_tune {
                              2: one launch, one workgroup per item, in queue order;
                              3: one launch per target group (per-leaf launches, no queue) */
     int waves_per_cu;     /* persistent grid = CUs * waves_per_cu (capped by occupancy) */
+    int split_tiles;      /* 0: a tile with fewer targets than lanes*T spreads each target over
+                             several lanes (default); -1: one lane per target always */
 } stkfmm_tune;
 
 typedef struct stkfmm_plan stkfmm_plan;

@@ -385,10 +385,19 @@ class Plan:
         if ng and gp[-1] > len(rb):
             raise ArgumentError("grp_rng_ptr points past the range arrays")
         ip = lambda a: None if a is None else a.ctypes.data_as(_capi.iptr)  # noqa: E731
+        if isinstance(tune, str):
+            if tune != "auto":
+                raise ArgumentError(f"tune: expected a dict, None or 'auto', got {tune!r}")
+            from .autotune import lookup, signature
+            csum = np.concatenate([[0], np.cumsum(re.astype(np.int64) - rb.astype(np.int64))])
+            per_group = csum[gp[1:]] - csum[gp[:-1]]
+            pairs = int(np.sum(per_group * (ge.astype(np.int64) - gb.astype(np.int64))))
+            tune = lookup(signature(uker, precision, n_trg, pairs, ng))
         t = None
         if tune is not None:
             t = StkTune(int(tune.get("targets_per_lane", 0)), int(tune.get("max_src_per_item", 0)),
-                        int(tune.get("persistent", 0)), int(tune.get("waves_per_cu", 0)))
+                        int(tune.get("persistent", 0)), int(tune.get("waves_per_cu", 0)),
+                        int(tune.get("split_tiles", 0)))
         h = ctypes.c_void_p()
         _check(lib().stkfmm_plan_create(ctypes.byref(h), int(uker), int(precision), int(n_src), int(n_trg), ng,
                                         ip(gb), ip(ge), ip(gl), ip(gp), ip(rb), ip(re),

@@ -232,7 +232,7 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
     if (n_groups > 0 && (!grp_trg_begin || !grp_trg_end || !grp_rng_ptr))
         return set_error(STKFMM_ERR_ARG, "group arrays are NULL");
 
-    stkfmm_tune tn{0, 0, 0, 0};
+    stkfmm_tune tn{0, 0, 0, 0, 0};
     if (tune)
         tn = *tune;
     const int T = tn.targets_per_lane ? tn.targets_per_lane : 2;
@@ -279,9 +279,10 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
     if (max_src <= 0 && persistent == 3) {
         max_src = 512; // per-group launches: split like a stand-alone direct sum of that group
     } else if (max_src <= 0) {
-        const long long target_items = 4LL * concurrent;
+        // about 16 items per concurrently resident wave keeps the tail of the cost-sorted queue short
+        const long long target_items = 16LL * concurrent;
         max_src = (total_cost + target_items - 1) / std::max(1LL, target_items);
-        max_src = std::max(512LL, ((max_src + kSrcTile - 1) / kSrcTile) * kSrcTile);
+        max_src = std::max(512LL, ((max_src + 511) / 512) * 512);
     }
     if (max_src < 1)
         max_src = 1;
@@ -307,7 +308,12 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
             tl.trg_begin = tb;
             tl.trg_count = std::min(TT, e - tb);
             tl.slot_begin = (int)items.size();
-            Item cur{tl.trg_begin, tl.trg_count, (int)ranges.size(), (int)ranges.size(), 0, level, 0, 0};
+            // lanes per target: the largest power of two (at most 16) that still fits the tile
+            int lsplit = 0;
+            if (tn.split_tiles >= 0)
+                while (lsplit < 4 && (tl.trg_count << (lsplit + 1)) <= TT)
+                    ++lsplit;
+            Item cur{tl.trg_begin, tl.trg_count, (int)ranges.size(), (int)ranges.size(), 0, level, 0, lsplit};
             long long cur_n = 0;
             for (int r = r0; r < r1; ++r) {
                 int sb = rng_src_begin[r];

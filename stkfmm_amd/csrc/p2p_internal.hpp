@@ -24,7 +24,9 @@ struct Item {
     int slot;       // partial-sum slot this item writes
     int level;      // octree level of the target group (mixed-mode scaling by 2^level)
     int cost;       // sources in the item (scheduling key)
-    int pad;
+    int lsplit;     // log2 of the lanes per target: a tile of n targets uses 2^lsplit lanes per
+                    // target when n * 2^lsplit <= 64*T; each of those lanes takes every
+                    // 2^lsplit-th staged source and the partial sums meet by wavefront shuffles
 };
 
 // a target tile and the partial-sum slots that cover it

@@ -25,15 +25,30 @@
 
 namespace stkp2p {
 
+// STKP2P_MIN_WAVES_PER_EU (build option): ask the compiler to keep at least this many waves of the
+// item kernel resident per SIMD (register budget 512 / waves).  0 leaves the choice to it.
+#ifndef STKP2P_MIN_WAVES_PER_EU
+#define STKP2P_MIN_WAVES_PER_EU 0
+#endif
+#if STKP2P_MIN_WAVES_PER_EU > 0
+#define STKP2P_ITEM_ATTR __attribute__((amdgpu_waves_per_eu(STKP2P_MIN_WAVES_PER_EU, 8)))
+#else
+#define STKP2P_ITEM_ATTR
+#endif
+
 template <int UK, int PREC, int T>
-__global__ void __launch_bounds__(kWave) p2p_items_kernel(ItemArgs a) {
+__global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemArgs a) {
     using K = typename UkerOf<UK>::type;
-    using CT = typename std::conditional<PREC == 0, double, float>::type;
     constexpr int KL = K::KL, KT = K::KT, KS = K::KS, KS1 = K::KS1;
     constexpr int TT = kWave * T;
 
+    // FP64: values as [KL][tile] (two consecutive sources per 16-byte read);
+    // mixed: FP32 values as 16-byte aligned rows [tile][KLP] (one source per few 16-byte reads,
+    // all at immediate offsets from one base address)
+    constexpr int KLP = ((KL + 3) / 4) * 4;
     __shared__ double s_x[kSrcTile], s_y[kSrcTile], s_z[kSrcTile];
-    __shared__ CT s_v[KL][kSrcTile];
+    __shared__ double s_vd[PREC == 0 ? KL : 1][kSrcTile];
+    __shared__ __attribute__((aligned(16))) float s_vf[PREC == 0 ? 1 : kSrcTile][KLP];
 
     const int lane = threadIdx.x;
     bool first = true;
@@ -55,11 +70,17 @@ __global__ void __launch_bounds__(kWave) p2p_items_kernel(ItemArgs a) {
 
         const Item item = a.items[it];
         const double lam = (PREC == 0) ? 1.0 : ldexp(1.0, item.level);
+        // lanes per target (uniform): lane = target slot * split + sub
+        const int ls = item.lsplit;
+        const int split = 1 << ls;
+        const int sub = lane & (split - 1);
+        const int slot0 = lane >> ls;
+        const int tstride = kWave >> ls;
 
         double tx[T], ty[T], tz[T];
 #pragma unroll
         for (int t = 0; t < T; ++t) {
-            const int i = lane + kWave * t;
+            const int i = slot0 + tstride * t;
             const long long idx = item.trg_begin + ((i < item.trg_count) ? i : item.trg_count - 1);
             tx[t] = a.tc.p[idx * a.tc.s] * lam;
             ty[t] = a.tc.p[idx * a.tc.s + a.tc.fs] * lam;
@@ -102,61 +123,84 @@ __global__ void __launch_bounds__(kWave) p2p_items_kernel(ItemArgs a) {
                     s_y[lane] = y;
                     s_z[lane] = z;
 #pragma unroll
-                    for (int k = 0; k < KL; ++k)
-                        s_v[k][lane] = (CT)vst[k];
+                    for (int k = 0; k < KL; ++k) {
+                        if (PREC == 0)
+                            s_vd[k][lane] = vst[k];
+                        else
+                            s_vf[lane][k] = (float)vst[k];
+                    }
                 }
                 __syncthreads();
-                if (PREC == 0) {
+                // sources of this tile: every lane takes all of them (split == 1, uniform loop with
+                // immediate LDS offsets) or every split-th one starting at its sub-lane
+                auto tile_pairs = [&](auto strided) {
+                    constexpr bool STR = decltype(strided)::value;
+                    const int j_first = STR ? sub : 0;
+                    const int j_step = STR ? split : 1;
+                    if (PREC == 0) {
 #pragma unroll 2
-                    for (int j = 0; j < n; ++j) {
-                        const double sx = s_x[j], sy = s_y[j], sz = s_z[j];
-                        double f[KL];
-#pragma unroll
-                        for (int k = 0; k < KL; ++k)
-                            f[k] = (double)s_v[k][j];
-#pragma unroll
-                        for (int t = 0; t < T; ++t)
-                            K::template eval<double>(acc[t], tx[t] - sx, ty[t] - sy, tz[t] - sz, f);
-                    }
-                } else {
-                    for (int j0 = 0; j0 < n; j0 += kMixedAccBlock) {
-                        const int j1 = (n - j0 < kMixedAccBlock) ? n : j0 + kMixedAccBlock;
-                        float part[T][KT];
-#pragma unroll
-                        for (int t = 0; t < T; ++t)
-#pragma unroll
-                            for (int k = 0; k < KT; ++k)
-                                part[t][k] = 0.0f;
-#pragma unroll 2
-                        for (int j = j0; j < j1; ++j) {
+                        for (int j = j_first; j < n; j += j_step) {
                             const double sx = s_x[j], sy = s_y[j], sz = s_z[j];
-                            float f[KL];
+                            double f[KL];
 #pragma unroll
                             for (int k = 0; k < KL; ++k)
-                                f[k] = (float)s_v[k][j];
+                                f[k] = s_vd[k][j];
 #pragma unroll
                             for (int t = 0; t < T; ++t)
-                                K::template eval<float>(part[t], (float)(tx[t] - sx), (float)(ty[t] - sy),
-                                                        (float)(tz[t] - sz), f);
+                                K::template eval<double>(acc[t], tx[t] - sx, ty[t] - sy, tz[t] - sz, f);
                         }
+                    } else {
+                        for (int j0 = 0; j0 < n; j0 += kMixedAccBlock) {
+                            const int j1 = (n - j0 < kMixedAccBlock) ? n : j0 + kMixedAccBlock;
+                            float part[T][KT];
 #pragma unroll
-                        for (int t = 0; t < T; ++t)
+                            for (int t = 0; t < T; ++t)
 #pragma unroll
-                            for (int k = 0; k < KT; ++k)
-                                acc[t][k] += (double)part[t][k];
+                                for (int k = 0; k < KT; ++k)
+                                    part[t][k] = 0.0f;
+#pragma unroll 2
+                            for (int j = j0 + j_first; j < j1; j += j_step) {
+                                const double sx = s_x[j], sy = s_y[j], sz = s_z[j];
+                                float f[KL];
+#pragma unroll
+                                for (int k = 0; k < KL; ++k)
+                                    f[k] = s_vf[j][k];
+#pragma unroll
+                                for (int t = 0; t < T; ++t)
+                                    K::template eval<float>(part[t], (float)(tx[t] - sx), (float)(ty[t] - sy),
+                                                            (float)(tz[t] - sz), f);
+                            }
+#pragma unroll
+                            for (int t = 0; t < T; ++t)
+#pragma unroll
+                                for (int k = 0; k < KT; ++k)
+                                    acc[t][k] += (double)part[t][k];
+                        }
                     }
-                }
+                };
+                if (ls == 0)
+                    tile_pairs(std::false_type{});
+                else
+                    tile_pairs(std::true_type{});
             }
         }
 
+        // lanes that share a target add their partial sums (butterfly over the low ls lane bits)
+        for (int off = 1; off < split; off <<= 1) {
+#pragma unroll
+            for (int t = 0; t < T; ++t)
+#pragma unroll
+                for (int k = 0; k < KT; ++k)
+                    acc[t][k] += __shfl_xor(acc[t][k], off);
+        }
         double *dst = a.partial + (long long)item.slot * KT * TT;
 #pragma unroll
         for (int k = 0; k < KT; ++k) {
-            const double osc = (PREC == 0) ? 1.0 : ldexp(1.0, K::deg(k) * item.level);
+            const double osc = ((PREC == 0) ? 1.0 : ldexp(1.0, K::deg(k) * item.level)) * K::post(k);
 #pragma unroll
             for (int t = 0; t < T; ++t) {
-                const int i = lane + kWave * t;
-                if (i < item.trg_count)
+                const int i = slot0 + tstride * t;
+                if (sub == 0 && i < item.trg_count)
                     dst[(long long)k * TT + i] = acc[t][k] * osc;
             }
         }

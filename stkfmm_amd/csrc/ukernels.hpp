@@ -14,32 +14,48 @@
 //   stage(in, out)  per-source precomputation in FP64.  Only sub-expressions the reference
 //         evaluates from source values alone are hoisted here (e.g. sxy + syx, a*a/3), each with
 //         the same operands and operation, so hoisting does not change a single rounding,
-//   eval<R>(u, dx, dy, dz, f)  u += K(r, f) without the scale factor, r = x_trg - x_src, in the
-//         algebraic order of the reference uKerEval (FMA contraction is left to the compiler).
+//   eval<R>(u, dx, dy, dz, f)  u += K(r, f) / post(k) without the scale factor, r = x_trg - x_src,
+//   post(k) constant factor applied to accumulated output k once per work item (so that per-pair
+//         constants such as the 2 of the pressure leave the inner loop).
+// The hot kernels (Stokeslet, PVel single/double/fused, regularised Stokeslet, RPY, Laplace PGrad)
+// evaluate the reference expression in an equivalent form with fewer operations (common factors
+// pulled out, constants folded into staged source values or post(), each term accumulated by its
+// own fused multiply-add); each such rewrite is stated next to its code.  The remaining
+// kernels follow the reference's algebraic order directly.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #ifndef STKP2P_FP64_NEWTON
-#define STKP2P_FP64_NEWTON 2
+#define STKP2P_FP64_NEWTON 1
 #endif
 
 namespace stkp2p {
 
 // ---- masked reciprocal square root ------------------------------------------------------------
-// approx_rsqrt(r2, r2 > 0) of the reference: 0 for coincident points.  v_rsq_{f64,f32} seeds,
-// refined by Newton steps y <- y + y (1 - r2 y^2)/2 written as e = 0.5 - (r2/2) y^2, y = y + y e.
+// approx_rsqrt(r2, r2 > 0) of the reference: 0 for coincident points.
+// FP64: v_rsq_f64 seed (measured on gfx950: 24.2 correct bits over r2 in [1e-30, 1e3]) refined by
+// STKP2P_FP64_NEWTON Newton steps y <- y + y (1 - r2 y^2)/2 (one step: 47.8 bits, max relative
+// error 4e-15; two: 52.4 bits), written as e = 0.5 - (r2/2) y^2, y = y + y e.
+// The mask acts on the seed: r2 = +0 gives v_rsq_f64 = +inf = 0x7ff0000000000000, whose low word is
+// already 0, so clearing the high word makes the seed +0 and every Newton step keeps it 0 -- one
+// 32-bit select instead of a 64-bit select after the refinement.  (r2 is a sum of squares: +0 or
+// positive, never -0 or negative.)
 __device__ __forceinline__ double rsqrt_masked(double r2) {
     double y = __builtin_amdgcn_rsq(r2);
+    unsigned long long bits = __builtin_bit_cast(unsigned long long, y);
+    const unsigned hi = (r2 > 0.0) ? (unsigned)(bits >> 32) : 0u;
+    y = __builtin_bit_cast(double, ((unsigned long long)hi << 32) | (bits & 0xffffffffull));
     const double hr = 0.5 * r2;
 #pragma unroll
     for (int i = 0; i < STKP2P_FP64_NEWTON; ++i) {
         const double e = __builtin_fma(-hr, y * y, 0.5);
         y = __builtin_fma(y, e, y);
     }
-    return (r2 > 0.0) ? y : 0.0;
+    return y;
 }
 
+// FP32: v_rsq_f32 seed (measured: 23.2 bits) and one Newton step.
 __device__ __forceinline__ float rsqrt_masked(float r2) {
     float y = __builtin_amdgcn_rsqf(r2);
     const float hr = 0.5f * r2;
@@ -58,6 +74,7 @@ struct UkLaplaceP {
     static constexpr int KS = 1, KS1 = 1, KL = 1, KT = 1;
     static constexpr unsigned SRC_SCALE_MASK = 0u;
     __host__ __device__ static constexpr int deg(int) { return 1; }
+    __host__ __device__ static constexpr double post(int k) { return ((void)k, 1.0); }
     __device__ static void stage(const double *in, double *o) { o[0] = in[0]; }
     template <class R>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
@@ -72,6 +89,7 @@ struct UkLaplaceDipoleP {
     static constexpr int KS = 3, KS1 = 3, KL = 3, KT = 1;
     static constexpr unsigned SRC_SCALE_MASK = 0x7u;
     __host__ __device__ static constexpr int deg(int) { return 1; }
+    __host__ __device__ static constexpr double post(int k) { return ((void)k, 1.0); }
     __device__ static void stage(const double *in, double *o) { o[0] = in[0]; o[1] = in[1]; o[2] = in[2]; }
     template <class R>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
@@ -83,21 +101,25 @@ struct UkLaplaceDipoleP {
     }
 };
 
-// laplace_pgrad, LaplaceLayerKernel.hpp:84-95
+// laplace_pgrad, LaplaceLayerKernel.hpp:84-95: p += q/r, grad p -= q r/r^3.
+// Rewrite: the reference's sv*r2 (= q rinv^3 r^2) is q*rinv; the gradient is accumulated with a
+// positive sign and negated by post().
 struct UkLaplacePGrad {
     static constexpr int KS = 1, KS1 = 1, KL = 1, KT = 4;
     static constexpr unsigned SRC_SCALE_MASK = 0u;
     __host__ __device__ static constexpr int deg(int k) { return k == 0 ? 1 : 2; }
+    __host__ __device__ static constexpr double post(int k) { return k == 0 ? 1.0 : -1.0; }
     __device__ static void stage(const double *in, double *o) { o[0] = in[0]; }
     template <class R>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
         R rinv = rsqrt_masked(r2);
-        R sv = f[0] * rinv * rinv * rinv;
-        u[0] += sv * r2;
-        u[1] -= sv * dx;
-        u[2] -= sv * dy;
-        u[3] -= sv * dz;
+        R qr = f[0] * rinv;
+        R sv = qr * (rinv * rinv);
+        u[0] += qr;
+        u[1] += sv * dx;
+        u[2] += sv * dy;
+        u[3] += sv * dz;
     }
 };
 
@@ -106,6 +128,7 @@ struct UkLaplaceDipolePGrad {
     static constexpr int KS = 3, KS1 = 3, KL = 3, KT = 4;
     static constexpr unsigned SRC_SCALE_MASK = 0x7u;
     __host__ __device__ static constexpr int deg(int k) { return k == 0 ? 1 : 2; }
+    __host__ __device__ static constexpr double post(int k) { return ((void)k, 1.0); }
     __device__ static void stage(const double *in, double *o) { o[0] = in[0]; o[1] = in[1]; o[2] = in[2]; }
     template <class R>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
@@ -128,6 +151,7 @@ struct UkLaplacePGradGrad {
     static constexpr int KS = 1, KS1 = 1, KL = 1, KT = 10;
     static constexpr unsigned SRC_SCALE_MASK = 0u;
     __host__ __device__ static constexpr int deg(int k) { return k == 0 ? 1 : (k < 4 ? 2 : 3); }
+    __host__ __device__ static constexpr double post(int k) { return ((void)k, 1.0); }
     __device__ static void stage(const double *in, double *o) { o[0] = in[0]; }
     template <class R>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
@@ -156,6 +180,7 @@ struct UkLaplaceDipolePGradGrad {
     static constexpr int KS = 3, KS1 = 3, KL = 3, KT = 10;
     static constexpr unsigned SRC_SCALE_MASK = 0x7u;
     __host__ __device__ static constexpr int deg(int k) { return k == 0 ? 1 : (k < 4 ? 2 : 3); }
+    __host__ __device__ static constexpr double post(int k) { return ((void)k, 1.0); }
     __device__ static void stage(const double *in, double *o) { o[0] = in[0]; o[1] = in[1]; o[2] = in[2]; }
     template <class R>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
@@ -191,6 +216,7 @@ struct UkStokesVel {
     static constexpr int KS = 3, KS1 = 3, KL = 3, KT = 3;
     static constexpr unsigned SRC_SCALE_MASK = 0u;
     __host__ __device__ static constexpr int deg(int) { return 1; }
+    __host__ __device__ static constexpr double post(int k) { return ((void)k, 1.0); }
     __device__ static void stage(const double *in, double *o) { o[0] = in[0]; o[1] = in[1]; o[2] = in[2]; }
     template <class R>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
@@ -199,17 +225,23 @@ struct UkStokesVel {
         R rinv3 = rinv * rinv * rinv;
         R fdotr = f[0] * dx + f[1] * dy + f[2] * dz;
         R c = fdotr * rinv3;
-        u[0] += f[0] * rinv + dx * c;
-        u[1] += f[1] * rinv + dy * c;
-        u[2] += f[2] * rinv + dz * c;
+        u[0] += f[0] * rinv;
+        u[0] += dx * c;
+        u[1] += f[1] * rinv;
+        u[1] += dy * c;
+        u[2] += f[2] * rinv;
+        u[2] += dz * c;
     }
 };
 
-// stokes_pvel, StokesSingleLayerKernel.hpp:33-46; source [fx, fy, fz, TrD]
+// stokes_pvel, StokesSingleLayerKernel.hpp:33-46; source [fx, fy, fz, TrD].
+//   p = 2 (f.r)/r^3,  u_i = (r^2 f_i + r_i (f.r - TrD))/r^3 = f_i/r + r_i (f.r - TrD)/r^3.
+// Rewrite: r^2/r^3 = 1/r; the factor 2 of p is applied by post().
 struct UkStokesPVel {
     static constexpr int KS = 4, KS1 = 4, KL = 4, KT = 4;
     static constexpr unsigned SRC_SCALE_MASK = 0x8u;
     __host__ __device__ static constexpr int deg(int k) { return k == 0 ? 2 : 1; }
+    __host__ __device__ static constexpr double post(int k) { return k == 0 ? 2.0 : 1.0; }
     __device__ static void stage(const double *in, double *o) {
         o[0] = in[0]; o[1] = in[1]; o[2] = in[2]; o[3] = in[3];
     }
@@ -217,63 +249,71 @@ struct UkStokesPVel {
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
         R rinv = rsqrt_masked(r2);
-        R rinv3 = rinv * rinv * rinv;
-        R commonCoeff = f[0] * dx + f[1] * dy + f[2] * dz;
-        const R two = R(2.0);
-        u[0] += two * commonCoeff * rinv3;
-        commonCoeff -= f[3];
-        u[1] += rinv3 * (r2 * f[0] + dx * commonCoeff);
-        u[2] += rinv3 * (r2 * f[1] + dy * commonCoeff);
-        u[3] += rinv3 * (r2 * f[2] + dz * commonCoeff);
+        R rinv3 = rinv * (rinv * rinv);
+        R cc = f[0] * dx + f[1] * dy + f[2] * dz;
+        u[0] += cc * rinv3;
+        R b = (cc - f[3]) * rinv3;
+        u[1] += f[0] * rinv;
+        u[1] += dx * b;
+        u[2] += f[1] * rinv;
+        u[2] += dy * b;
+        u[3] += f[2] * rinv;
+        u[3] += dz * b;
     }
 };
 
 // stokes_doublepvel, StokesDoubleLayerKernel.hpp:31-56; source D row-major [Dxx..Dzz].
-// Staged: Dxx, Dyy, Dzz, Dxy+Dyx, Dxz+Dzx, Dyz+Dzy, Dxx+Dyy+Dzz (the reference's own source-only
-// sub-expressions, :46-48,51).
+//   Q = r.D.r,  p = 2 (tr D/r^3 - 3 Q/r^5),  u_i = -3 r_i Q/r^5.
+// Staged: -3 Dxx, -3 Dyy, -3 Dzz, -3 (Dxy+Dyx), -3 (Dxz+Dzx), -3 (Dyz+Dzy), Dxx+Dyy+Dzz (the
+// reference's source-only sub-expressions, :46-48,51, with its factor -3 of :49 folded in).
+// Rewrite: q = -3 Q/r^5 by Horner in dx, dy, dz; p/2 = tr D/r^3 + q, u_i = r_i q; post() doubles p.
 struct UkStokesDoublePVel {
     static constexpr int KS = 9, KS1 = 9, KL = 7, KT = 4;
     static constexpr unsigned SRC_SCALE_MASK = 0x1FFu;
     __host__ __device__ static constexpr int deg(int k) { return k == 0 ? 2 : 1; }
+    __host__ __device__ static constexpr double post(int k) { return k == 0 ? 2.0 : 1.0; }
     __device__ static void stage(const double *in, double *o) {
-        o[0] = in[0];
-        o[1] = in[4];
-        o[2] = in[8];
-        o[3] = in[1] + in[3];
-        o[4] = in[2] + in[6];
-        o[5] = in[5] + in[7];
+        o[0] = -3.0 * in[0];
+        o[1] = -3.0 * in[4];
+        o[2] = -3.0 * in[8];
+        o[3] = -3.0 * (in[1] + in[3]);
+        o[4] = -3.0 * (in[2] + in[6]);
+        o[5] = -3.0 * (in[5] + in[7]);
         o[6] = in[0] + in[4] + in[8];
     }
+    // -3 Q/r^5 from staged values
     template <class R>
-    __device__ static void eval_staged(R *u, R dx, R dy, R dz, R r2, R rinv, const R *g) {
-        R rinv3 = rinv * rinv * rinv;
-        R rinv5 = rinv3 * rinv * rinv;
-        const R two = R(2.0);
-        R commonCoeff = g[0] * dx * dx + g[1] * dy * dy + g[2] * dz * dz;
-        commonCoeff += g[3] * dx * dy;
-        commonCoeff += g[4] * dx * dz;
-        commonCoeff += g[5] * dy * dz;
-        commonCoeff *= R(-3.0) * rinv5;
-        u[0] += two * (commonCoeff + r2 * rinv5 * g[6]);
-        u[1] += dx * commonCoeff;
-        u[2] += dy * commonCoeff;
-        u[3] += dz * commonCoeff;
+    __device__ static R q5(R dx, R dy, R dz, R rinv5, const R *g) {
+        R a1 = g[0] * dx + g[3] * dy + g[4] * dz;
+        R a2 = g[1] * dy + g[5] * dz;
+        R a3 = g[2] * dz;
+        return (dx * a1 + dy * a2 + dz * a3) * rinv5;
     }
     template <class R>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *g) {
         R r2 = dx * dx + dy * dy + dz * dz;
         R rinv = rsqrt_masked(r2);
-        eval_staged(u, dx, dy, dz, r2, rinv, g);
+        R rinv2 = rinv * rinv;
+        R rinv3 = rinv2 * rinv;
+        R q = q5(dx, dy, dz, rinv3 * rinv2, g);
+        u[0] += q;
+        u[0] += g[6] * rinv3;
+        u[1] += dx * q;
+        u[2] += dy * q;
+        u[3] += dz * q;
     }
 };
 
 // stokes_pvel + stokes_doublepvel at one source point (StokesLayerKernel.hpp:44-50 with SL points =
-// DL points); the displacement, r2 and rsqrt are shared by the two layers.  Source values:
-// first array [fx, fy, fz, TrD], second array D row-major.
+// DL points): displacement, r2 and rsqrt are shared by the two layers.  Source values: first array
+// [fx, fy, fz, TrD], second array D row-major (staged as in UkStokesDoublePVel).  With cc = f.r and
+// q = -3 Q/r^5:  p/2 = (cc + tr D)/r^3 + q,   u_i = f_i/r + r_i ((cc - TrD)/r^3 + q);  post()
+// doubles p.
 struct UkStokesPVelFused {
     static constexpr int KS = 13, KS1 = 4, KL = 11, KT = 4;
     static constexpr unsigned SRC_SCALE_MASK = 0x1FF8u;
     __host__ __device__ static constexpr int deg(int k) { return k == 0 ? 2 : 1; }
+    __host__ __device__ static constexpr double post(int k) { return k == 0 ? 2.0 : 1.0; }
     __device__ static void stage(const double *in, double *o) {
         o[0] = in[0]; o[1] = in[1]; o[2] = in[2]; o[3] = in[3];
         UkStokesDoublePVel::stage(in + 4, o + 4);
@@ -282,24 +322,30 @@ struct UkStokesPVelFused {
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
         R rinv = rsqrt_masked(r2);
-        R rinv3 = rinv * rinv * rinv;
-        R commonCoeff = f[0] * dx + f[1] * dy + f[2] * dz;
-        const R two = R(2.0);
-        u[0] += two * commonCoeff * rinv3;
-        commonCoeff -= f[3];
-        u[1] += rinv3 * (r2 * f[0] + dx * commonCoeff);
-        u[2] += rinv3 * (r2 * f[1] + dy * commonCoeff);
-        u[3] += rinv3 * (r2 * f[2] + dz * commonCoeff);
-        UkStokesDoublePVel::eval_staged(u, dx, dy, dz, r2, rinv, f + 4);
+        R rinv2 = rinv * rinv;
+        R rinv3 = rinv2 * rinv;
+        R cc = f[0] * dx + f[1] * dy + f[2] * dz;
+        R q = UkStokesDoublePVel::q5(dx, dy, dz, rinv3 * rinv2, f + 4);
+        u[0] += q;
+        u[0] += (cc + f[10]) * rinv3;
+        R b = (cc - f[3]) * rinv3 + q;
+        u[1] += f[0] * rinv;
+        u[1] += dx * b;
+        u[2] += f[1] * rinv;
+        u[2] += dy * b;
+        u[3] += f[2] * rinv;
+        u[3] += dz * b;
     }
 };
 
 // stokes_regvel, StokesRegSingleLayerKernel.hpp:23-40; source [fx, fy, fz, eps].
-// Staged: fx, fy, fz, eps*eps.  The mask is on the regularised r2 = |d|^2 + eps^2.
+//   rho2 = |d|^2 + eps^2,  u_i = ((rho2 + eps^2) f_i + d_i (f.d))/rho^3.
+// Staged: fx, fy, fz, eps*eps.  The mask is on the regularised rho2.
 struct UkStokesRegVel {
     static constexpr int KS = 4, KS1 = 4, KL = 4, KT = 3;
     static constexpr unsigned SRC_SCALE_MASK = 0x8u;
     __host__ __device__ static constexpr int deg(int) { return 1; }
+    __host__ __device__ static constexpr double post(int k) { return ((void)k, 1.0); }
     __device__ static void stage(const double *in, double *o) {
         o[0] = in[0]; o[1] = in[1]; o[2] = in[2]; o[3] = in[3] * in[3];
     }
@@ -309,91 +355,115 @@ struct UkStokesRegVel {
         R r2 = dx * dx + dy * dy + dz * dz + reg2;
         R rinv = rsqrt_masked(r2);
         R rinv3 = rinv * rinv * rinv;
-        R r2reg2 = r2 + reg2;
-        R commonCoeff = f[0] * dx + f[1] * dy + f[2] * dz;
-        u[0] += rinv3 * (r2reg2 * f[0] + dx * commonCoeff);
-        u[1] += rinv3 * (r2reg2 * f[1] + dy * commonCoeff);
-        u[2] += rinv3 * (r2reg2 * f[2] + dz * commonCoeff);
+        R a = (r2 + reg2) * rinv3;
+        R b = (f[0] * dx + f[1] * dy + f[2] * dz) * rinv3;
+        u[0] += f[0] * a;
+        u[0] += dx * b;
+        u[1] += f[1] * a;
+        u[1] += dy * b;
+        u[2] += f[2] * a;
+        u[2] += dz * b;
     }
 };
 
-// rpy_u, RPYKernel.hpp:23-44; source [fx, fy, fz, a].  Staged: fx, fy, fz, one_over_three*a*a.
+// rpy_u, RPYKernel.hpp:23-44; source [fx, fy, fz, a].
+//   u_i = f_i/r + r_i (f.r)/r^3 + (a^2/3) (f_i/r^3 - 3 (f.r) r_i/r^5)
+//       = f_i (1/r + beta/r^3) + r_i ((f.r)/r^3 - 3 beta (f.r)/r^5),  beta = one_over_three a^2.
+// Staged: fx, fy, fz, beta, -3 beta.
 struct UkRpyU {
-    static constexpr int KS = 4, KS1 = 4, KL = 4, KT = 3;
+    static constexpr int KS = 4, KS1 = 4, KL = 5, KT = 3;
     static constexpr unsigned SRC_SCALE_MASK = 0x8u;
     __host__ __device__ static constexpr int deg(int) { return 1; }
+    __host__ __device__ static constexpr double post(int k) { return ((void)k, 1.0); }
     __device__ static void stage(const double *in, double *o) {
-        o[0] = in[0]; o[1] = in[1]; o[2] = in[2]; o[3] = STKP2P_RPY_ONE_OVER_THREE * in[3] * in[3];
+        o[0] = in[0]; o[1] = in[1]; o[2] = in[2];
+        o[3] = STKP2P_RPY_ONE_OVER_THREE * in[3] * in[3];
+        o[4] = -3.0 * o[3];
     }
     template <class R>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
         R rinv = rsqrt_masked(r2);
-        R rinv3 = rinv * rinv * rinv;
-        R rinv5 = rinv3 * rinv * rinv;
-        const R three = R(3.0);
-        R fdotr = f[0] * dx + f[1] * dy + f[2] * dz;
-        R three_fdotr_rinv5 = three * fdotr * rinv5;
-        const R a2_over_three = f[3];
-        u[0] += (r2 * f[0] + dx * fdotr) * rinv3 + a2_over_three * (f[0] * rinv3 - three_fdotr_rinv5 * dx);
-        u[1] += (r2 * f[1] + dy * fdotr) * rinv3 + a2_over_three * (f[1] * rinv3 - three_fdotr_rinv5 * dy);
-        u[2] += (r2 * f[2] + dz * fdotr) * rinv3 + a2_over_three * (f[2] * rinv3 - three_fdotr_rinv5 * dz);
+        R rinv2 = rinv * rinv;
+        R rinv3 = rinv2 * rinv;
+        R fr = f[0] * dx + f[1] * dy + f[2] * dz;
+        R g3 = fr * rinv3;
+        R A = f[3] * rinv3 + rinv;
+        R B = f[4] * (g3 * rinv2) + g3;
+        u[0] += f[0] * A;
+        u[0] += dx * B;
+        u[1] += f[1] * A;
+        u[1] += dy * B;
+        u[2] += f[2] * A;
+        u[2] += dz * B;
     }
 };
 
-// rpy_ulapu, RPYKernel.hpp:58-89; source [fx, fy, fz, a].  Staged as UkRpyU.
+// rpy_ulapu, RPYKernel.hpp:58-89; source [fx, fy, fz, a].  With c_i = f_i/r^3 - 3 (f.r) r_i/r^5:
+//   u_i = f_i/r + r_i (f.r)/r^3 + beta c_i,  lap u_i = 2 c_i  (beta = one_over_three a^2)
+// evaluated as u_i = f_i (1/r + beta/r^3) + r_i (g3 - 3 beta g5), lap u_i/2 = f_i/r^3 - 3 g5 r_i
+// (g3 = (f.r)/r^3, g5 = (f.r)/r^5); post() doubles lap u.  Staged as UkRpyU.
 struct UkRpyULapU {
-    static constexpr int KS = 4, KS1 = 4, KL = 4, KT = 6;
+    static constexpr int KS = 4, KS1 = 4, KL = 5, KT = 6;
     static constexpr unsigned SRC_SCALE_MASK = 0x8u;
     __host__ __device__ static constexpr int deg(int k) { return k < 3 ? 1 : 3; }
+    __host__ __device__ static constexpr double post(int k) { return k < 3 ? 1.0 : 2.0; }
     __device__ static void stage(const double *in, double *o) { UkRpyU::stage(in, o); }
     template <class R>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
         R rinv = rsqrt_masked(r2);
-        R rinv3 = rinv * rinv * rinv;
-        R rinv5 = rinv3 * rinv * rinv;
-        const R two = R(2.0);
-        const R three = R(3.0);
-        R fdotr = f[0] * dx + f[1] * dy + f[2] * dz;
-        R fdotr_rinv3 = fdotr * rinv3;
-        R three_fdotr_rinv5 = three * fdotr * rinv5;
-        const R a2_over_three = f[3];
-        R cx = f[0] * rinv3 - three_fdotr_rinv5 * dx;
-        R cy = f[1] * rinv3 - three_fdotr_rinv5 * dy;
-        R cz = f[2] * rinv3 - three_fdotr_rinv5 * dz;
-        u[0] += f[0] * rinv + dx * fdotr_rinv3 + a2_over_three * cx;
-        u[1] += f[1] * rinv + dy * fdotr_rinv3 + a2_over_three * cy;
-        u[2] += f[2] * rinv + dz * fdotr_rinv3 + a2_over_three * cz;
-        u[3] += two * cx;
-        u[4] += two * cy;
-        u[5] += two * cz;
+        R rinv2 = rinv * rinv;
+        R rinv3 = rinv2 * rinv;
+        R fr = f[0] * dx + f[1] * dy + f[2] * dz;
+        R g3 = fr * rinv3;
+        R g5 = g3 * rinv2;
+        R A = f[3] * rinv3 + rinv;
+        R B = f[4] * g5 + g3;
+        R m = R(-3.0) * g5;
+        u[0] += f[0] * A;
+        u[0] += dx * B;
+        u[1] += f[1] * A;
+        u[1] += dy * B;
+        u[2] += f[2] * A;
+        u[2] += dz * B;
+        u[3] += f[0] * rinv3;
+        u[3] += dx * m;
+        u[4] += f[1] * rinv3;
+        u[4] += dy * m;
+        u[5] += f[2] * rinv3;
+        u[5] += dz * m;
     }
 };
 
-// stk_ulapu, RPYKernel.hpp:103-128; source [fx, fy, fz]
+// stk_ulapu, RPYKernel.hpp:103-128; source [fx, fy, fz]: rpy_ulapu with beta = 0.
 struct UkStkULapU {
     static constexpr int KS = 3, KS1 = 3, KL = 3, KT = 6;
     static constexpr unsigned SRC_SCALE_MASK = 0u;
     __host__ __device__ static constexpr int deg(int k) { return k < 3 ? 1 : 3; }
+    __host__ __device__ static constexpr double post(int k) { return k < 3 ? 1.0 : 2.0; }
     __device__ static void stage(const double *in, double *o) { o[0] = in[0]; o[1] = in[1]; o[2] = in[2]; }
     template <class R>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
         R rinv = rsqrt_masked(r2);
-        R rinv3 = rinv * rinv * rinv;
-        R rinv5 = rinv3 * rinv * rinv;
-        const R two = R(2.0);
-        const R three = R(3.0);
-        R fdotr = f[0] * dx + f[1] * dy + f[2] * dz;
-        R fdotr_rinv3 = fdotr * rinv3;
-        R three_fdotr_rinv5 = three * fdotr * rinv5;
-        u[0] += f[0] * rinv + dx * fdotr_rinv3;
-        u[1] += f[1] * rinv + dy * fdotr_rinv3;
-        u[2] += f[2] * rinv + dz * fdotr_rinv3;
-        u[3] += two * (f[0] * rinv3 - three_fdotr_rinv5 * dx);
-        u[4] += two * (f[1] * rinv3 - three_fdotr_rinv5 * dy);
-        u[5] += two * (f[2] * rinv3 - three_fdotr_rinv5 * dz);
+        R rinv2 = rinv * rinv;
+        R rinv3 = rinv2 * rinv;
+        R fr = f[0] * dx + f[1] * dy + f[2] * dz;
+        R g3 = fr * rinv3;
+        R m = R(-3.0) * (g3 * rinv2);
+        u[0] += f[0] * rinv;
+        u[0] += dx * g3;
+        u[1] += f[1] * rinv;
+        u[1] += dy * g3;
+        u[2] += f[2] * rinv;
+        u[2] += dz * g3;
+        u[3] += f[0] * rinv3;
+        u[3] += dx * m;
+        u[4] += f[1] * rinv3;
+        u[4] += dy * m;
+        u[5] += f[2] * rinv3;
+        u[5] += dz * m;
     }
 };
 

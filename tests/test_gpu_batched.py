@@ -62,7 +62,8 @@ def test_small_tree_all_leaves(uk, prec):
 
 @pytest.mark.parametrize("tune", [dict(targets_per_lane=1), dict(targets_per_lane=2, persistent=2),
                                   dict(targets_per_lane=4, max_src_per_item=300),
-                                  dict(targets_per_lane=2, max_src_per_item=64, waves_per_cu=1)])
+                                  dict(targets_per_lane=2, max_src_per_item=64, waves_per_cu=1),
+                                  dict(targets_per_lane=2, split_tiles=-1), dict(targets_per_lane=4, split_tiles=0)])
 def test_tuning_knobs_do_not_change_results(tune):
     say(f"tune {tune}")
     g = np.random.default_rng(7)
@@ -221,3 +222,26 @@ def test_workload5_instances_full_size():
         assert l2 < FP64_TOL and mx < FP64_TOL
         l2, mx = O.max_error(outs[1][a:b], ref)
         assert l2 < MIXED_TOL and mx < MIXED_TOL
+
+
+def test_autotuner_chooses_a_candidate_and_caches(tmp_path, monkeypatch):
+    from stkfmm_amd.autotune import autotune, lookup, signature
+    say("autotune")
+    monkeypatch.setenv("STKFMM_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    g = np.random.default_rng(31)
+    x = g.random((8000, 3))
+    tr, xs, sl, dl = sorted_problem(x, 150, 13, 32)
+    d_x, d_sl, d_dl = to_dev(xs), to_dev(sl), to_dev(dl)
+    scratch = zeros(tr.n, 4)
+    cands = [dict(targets_per_lane=1, max_src_per_item=1024, persistent=1),
+             dict(targets_per_lane=2, max_src_per_item=2048, persistent=2)]
+    run = dict(src_coord=d_x, src_value=d_sl, trg_coord=d_x, trg_value=scratch, src_value2=d_dl)
+    tune, timings = autotune(tr, 13, 1, run, candidates=cands, reps=2)
+    assert tune in cands and len(timings) == 2
+    assert lookup(signature(13, 1, tr.n, tr.pairs, tr.n_leaves)) == tune
+    # a plan built with tune="auto" finds the cached choice and gives oracle-parity results
+    plan = S.Plan.from_tree(tr, 13, 1, tune="auto")
+    out = zeros(tr.n, 4)
+    plan.execute(d_x, d_sl, d_x, out, src_value2=d_dl)
+    l2, mx = O.max_error(host(out), oracle_leaves(tr, 13, xs, sl, dl, None))
+    assert l2 < MIXED_TOL and mx < MIXED_TOL
