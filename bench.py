@@ -1,18 +1,22 @@
-"""Benchmark: Stokes PVel P2P over the U-list of an adaptive octree on one MI355X.
+"""Benchmarks of the STKFMM P2P path on one MI355X.
 
-Workload (the headline metric's configuration): 1,000,000 points, half uniform in [0,1)^3 and half
-in 16 Gaussian clusters (sigma 0.01), leaf cap 1000; single layer f ~ N(0,1)^3 (TrD = 0) and double
-layer 9 x N(0,1) at every point (SL = DL = targets; one pair = both layers).  One step = one full
-U-list evaluation (all leaves, one batched persistent launch + the partial-sum reduction), inputs
-resident in HBM.
+Default (the headline metric): Stokes PVel P2P over the U-list of an adaptive octree --
+1,000,000 points, half uniform in [0,1)^3 and half in 16 Gaussian clusters (sigma 0.01), leaf cap
+1000; single layer f ~ N(0,1)^3 (TrD = 0) and double layer 9 x N(0,1) at every point (SL = DL =
+targets; one pair covers both layers).  One step = one full U-list evaluation (every leaf, one
+batched persistent launch + the partial-sum reduction) with inputs resident in HBM.  value = Gpair/s
+of the mixed-precision batched launch; beside it the FP64 batched launch, the plain FP64
+per-leaf-launch version (one launch per leaf), accuracy of each against the FP64 oracle, the
+roofline of the dominant kernel (HIP events on its stream) and the CPU baseline (the oracle's
+vectorised C restatement of the reference P2P on the host cores).
 
-Reported: value = Gpair/s of the mixed-precision batched launch (whole job, 1 GPU); beside it the
-FP64 batched launch and the plain FP64 per-leaf-launch version (one launch per leaf), accuracy of
-both modes against the FP64 oracle on sampled leaves, the roofline of the dominant kernel (timed with
-HIP events on its own stream) and the CPU baseline (the oracle's vectorised C restatement of the
-reference P2P on the host cores, on a bounded sample of leaves).
+Other workloads (--workload), each printing its own JSON line:
+  lapgrad-direct        8192 x 8192 Laplace PGrad direct sum (STKFMM::evaluateKernel), FP64
+  rpy-ulist             4M log-normal clustered points, leaf cap 2000, RPY, mixed (autotuned) and FP64
+  pvel-adjoint          65,536 x 65,536 Stokes PVel forward + adjoint (gradients of sum_t w_t.u_t)
+  stokeslet-instances   512 independent Stokeslet direct sums in one HIP-graph replay
 
-  python bench.py [--steps K] [--warmup W] [--gpus 1]
+  python bench.py [--workload W] [--steps K] [--warmup W] [--gpus 1]
 """
 import argparse
 import json
@@ -25,212 +29,48 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-METRIC = "Gpair-interactions/s StokesPVel P2P: FP64 vs mixed at ≤1e-6 rel err"
-FLOPS_PER_PAIR = 72            # stokes_pvel + stokes_doublepvel fused, algorithmic count
-FLOPS_FP64_MIXED = 7           # FP64 part in mixed mode (3 displacement subtractions + 4 accumulations)
-PEAK_FP64 = 78.6e12            # MI355X vector FP64, FLOP/s
-PEAK_FP32 = 157.3e12           # MI355X vector FP32 (packed), FLOP/s
-HBM_PEAK = 8.0e12
+PEAK_FP64 = 78.6e12   # MI355X vector FP64, FLOP/s
+PEAK_FP32 = 157.3e12  # MI355X vector FP32 (published, packed), FLOP/s
+# algorithmic FLOPs per pair and the FP64 share in mixed mode (displacement + accumulation)
+FLOPS = {"pvel_fused": (72, 7), "rpy": (56, 9), "stokeslet": (29, 6), "lapgrad": (20, 7)}
+NOTE = ("VALU-bound; peak = MI355X published vector FP64 78.6 / FP32 157.3 TFLOP/s combined by the "
+        "FP64/FP32 split of the algorithmic FLOPs; this build keeps packed FP32 disabled")
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="pvel-ulist",
+                    choices=["pvel-ulist", "lapgrad-direct", "rpy-ulist", "pvel-adjoint", "stokeslet-instances"])
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--points", type=int, default=1_000_000)
-    ap.add_argument("--leaf-cap", type=int, default=1000)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU work for the baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU work budget of the baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-per-leaf", action="store_true")
     ap.add_argument("--only", choices=["mixed", "fp64", "per-leaf"], default=None,
-                    help="run one variant only (for profiling); prints its own line")
+                    help="run one variant only (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device copy of known size (PMC calibration) and exit")
     return ap.parse_args()
 
 
-def main():
-    args = parse()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print(json.dumps({"error": "this build measures one GPU only: the P2P path runs on a single "
-                                   "MI355X (multi-GPU sharding is a design note, not a measured mode)",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
-
-    import torch
-
-    import stkfmm_amd as S
-    from stkfmm_amd import workloads as W
-
-    dev = torch.device("cuda:0")
-    if args.copy_calibration:
-        n = 1 << 27  # 1 GiB of doubles
-        a = torch.ones(n, dtype=torch.float64, device=dev)
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8}))
-        return
-
-    t0 = time.perf_counter()
-    x = W.pvel_points(args.points)
-    tree = S.Tree(x, args.leaf_cap)
-    xs = x[tree.perm]
-    sl, dl = W.pvel_values(tree.n)
-    setup_s = time.perf_counter() - t0
-    d_x = torch.from_numpy(xs).to(dev)
-    d_sl = torch.from_numpy(sl).to(dev)
-    d_dl = torch.from_numpy(dl).to(dev)
-    out = torch.zeros((tree.n, 4), dtype=torch.float64, device=dev)
-    pairs = tree.pairs
-    stream = torch.cuda.Stream()
-
-    from stkfmm_amd.autotune import autotune
-
-    def run_variant(prec, tune, steps, warmup):
-        if tune == "autotune":
-            scratch = torch.zeros((tree.n, 4), dtype=torch.float64, device=dev)
-            tune, _ = autotune(tree, S.UKER.STOKES_PVEL_FUSED, prec,
-                               dict(src_coord=d_x, src_value=d_sl, trg_coord=d_x, trg_value=scratch,
-                                    src_value2=d_dl, stream=stream))
-            del scratch
-        plan = S.Plan.from_tree(tree, S.UKER.STOKES_PVEL_FUSED, prec, tune=tune)
-        res = torch.zeros((tree.n, 4), dtype=torch.float64, device=dev)
-        with torch.cuda.stream(stream):
-            for _ in range(warmup):
-                plan.execute(d_x, d_sl, d_x, out, src_value2=d_dl, stream=stream)
-            torch.cuda.synchronize()
-            plan.set_timing(True)
-            t = time.perf_counter()
-            for _ in range(steps):
-                plan.execute(d_x, d_sl, d_x, out, src_value2=d_dl, stream=stream)
-            torch.cuda.synchronize()
-            wall = time.perf_counter() - t
-            item_ms, reduce_ms, count = plan.kernel_ms()
-            plan.set_timing(False)
-            plan.execute(d_x, d_sl, d_x, res, src_value2=d_dl, stream=stream)
-            torch.cuda.synchronize()
-        st = plan.stats()
-        assert st["pairs"] == pairs
-        return dict(wall_s=wall, ms_per_step=1e3 * wall / steps, gpair_s=pairs * steps / wall / 1e9,
-                    item_ms=item_ms, reduce_ms=reduce_ms, timed_launches=count, n_items=st["n_items"],
-                    n_tiles=st["n_tiles"], grid=st["grid"], tune=tune), res
-
-    variants = {}
-    results = {}
-    plan_order = ["mixed", "fp64", "per-leaf"] if args.only is None else [args.only]
-    for name in plan_order:
-        if name == "per-leaf" and args.no_per_leaf:
-            continue
-        prec = S.PREC.MIXED if name == "mixed" else S.PREC.FP64
-        # batched variants use the autotuner's choice (cached per workload class); the per-leaf
-        # baseline launches every leaf separately with the library's default tiles
-        tune = dict(persistent=3) if name == "per-leaf" else "autotune"
-        steps = args.steps if name != "per-leaf" else max(1, min(args.steps, 3))
-        warm = args.warmup if name != "per-leaf" else 1
-        variants[name], results[name] = run_variant(prec, tune, steps, warm)
-        print(f"# {name}: {variants[name]['gpair_s']:.1f} Gpair/s, {variants[name]['ms_per_step']:.3f} ms/step, "
-              f"item kernel {variants[name]['item_ms']:.3f} ms", file=sys.stderr, flush=True)
-
-    head = "mixed" if "mixed" in variants else plan_order[0]
-    hv = variants[head]
-
-    # roofline of the dominant kernel (the item kernel of the headline variant)
-    item_s = hv["item_ms"] * 1e-3
-    achieved = pairs * FLOPS_PER_PAIR / item_s
-    if head == "mixed":
-        peak = FLOPS_PER_PAIR / (FLOPS_FP64_MIXED / PEAK_FP64 + (FLOPS_PER_PAIR - FLOPS_FP64_MIXED) / PEAK_FP32)
-    else:
-        peak = PEAK_FP64
-    roofline = {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None,
-                "kernel": f"p2p_items_kernel<STOKES_PVEL_FUSED,{'MIXED' if head == 'mixed' else 'FP64'},"
-                          f"T={(hv['tune'] or {}).get('targets_per_lane', 2)}>",
-                "kernel_ms": hv["item_ms"], "flops_per_pair": FLOPS_PER_PAIR, "pairs_per_launch": pairs,
-                "note": "VALU-bound; peak uses MI355X published vector FP64 78.6 and packed FP32 157.3 TFLOP/s; "
-                        "this build keeps packed FP32 disabled, so FP32 runs at the FP64 VALU rate"}
-    traffic_file = os.path.join(ROOT, "profiles", "pvel_ulist_1M_mixed_traffic.json")
-    if head == "mixed" and os.path.exists(traffic_file):
-        with open(traffic_file) as f:
-            tr = json.load(f)
-        roofline["traffic"] = tr.get("hbm_bytes_per_launch")
-        roofline["traffic_source"] = "profiles/pvel_ulist_1M_mixed_traffic.json (rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE)"
-
-    # accuracy and CPU baseline on sampled leaves (oracle = checker and timed CPU restatement)
-    accuracy = None
-    cpu = None
-    if not args.no_cpu_baseline:
-        from oracle import oracle as O
-        threads = max(1, min(16, os.cpu_count() or 1))
-        g = np.random.default_rng(12345)
-        order = g.permutation(tree.n_leaves)
-        sz = (tree.leaf_end - tree.leaf_begin).astype(np.int64)
-        nsrc = np.array([int((tree.rng_end[tree.ulist_ptr[l]:tree.ulist_ptr[l + 1]] -
-                              tree.rng_begin[tree.ulist_ptr[l]:tree.ulist_ptr[l + 1]]).sum())
-                         for l in range(tree.n_leaves)], dtype=np.int64)
-        vals = np.concatenate([sl, dl], axis=1)
-        # evaluate random leaves in chunks until about cpu-seconds of CPU work has been done
-        ref = np.zeros((tree.n, 4))
-        done = []
-        cpu_s = 0.0
-        pos = 0
-        chunk = 4 * threads
-        while pos < tree.n_leaves and cpu_s < args.cpu_seconds:
-            sel_c = order[pos:pos + chunk]
-            pos += len(sel_c)
-            t = time.perf_counter()
-            ref = O.ranges(13, xs, vals, xs, tree.leaf_begin, tree.leaf_end, tree.ulist_ptr, tree.rng_begin,
-                           tree.rng_end, sel=sel_c, nthreads=threads, simd=True, trg_value=ref)
-            cpu_s += time.perf_counter() - t
-            done.append(sel_c)
-        sel = np.concatenate(done)
-        sample_pairs = int((sz[sel] * nsrc[sel]).sum())
-        rows = np.concatenate([np.arange(tree.leaf_begin[l], tree.leaf_end[l]) for l in sel])
-        cpu = {"value": sample_pairs / cpu_s / 1e9, "unit": "Gpair/s", "cores": threads, "kind": "port",
-               "sample": f"{len(sel)} of {tree.n_leaves} leaves (random), {rows.size} targets, "
-                         f"{sample_pairs:.3e} pairs, {cpu_s:.1f} s; FP64 C restatement of the reference "
-                         "uKerEval algebra, AVX2 lanes over targets, OpenMP over leaves "
-                         "(PVFMM SIMD P2P not buildable offline)",
-               "cpu_model": _cpu_model()}
-        accuracy = {}
-        for name, res in results.items():
-            got = res.cpu().numpy()[rows]
-            l2, mx = O.max_error(got, ref[rows])
-            accuracy[name] = {"errorL2": l2, "errorMaxRel": mx}
-
-    value = hv["gpair_s"]
-    line = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "Gpair/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": hv["ms_per_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64+f32 (mixed)" if head == "mixed" else "f64",
-        "data": "synthetic",
-        "config": {"workload": "stokes-pvel-ulist-1M-clustered-leafcap1000", "points": tree.n,
-                   "leaf_cap": args.leaf_cap, "leaves": tree.n_leaves, "pairs": pairs,
-                   "layers": "single+double layer fused (SL = DL = targets)", "precision": head,
-                   "setup_s": setup_s},
-        "variants": variants,
-        "accuracy_vs_oracle": accuracy,
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-    }
-    if "per-leaf" in variants and head == "mixed":
-        line["speedup_vs_fp64_per_leaf_launch"] = value / variants["per-leaf"]["gpair_s"]
-    if "fp64" in variants and head == "mixed":
-        line["speedup_vs_fp64_batched"] = value / variants["fp64"]["gpair_s"]
-    print(json.dumps(line), flush=True)
+def peak_for(kind, prec):
+    f, f64 = FLOPS[kind]
+    if prec == "fp64":
+        return PEAK_FP64
+    return f / (f64 / PEAK_FP64 + (f - f64) / PEAK_FP32)
 
 
-def _cpu_model():
+def roofline(kind, prec, pairs, kernel_ms, kernel_name):
+    f = FLOPS[kind][0]
+    achieved = pairs * f / (kernel_ms * 1e-3)
+    peak = peak_for(kind, prec)
+    return {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": None, "kernel": kernel_name, "kernel_ms": kernel_ms,
+            "flops_per_pair": f, "pairs_per_launch": pairs, "note": NOTE}
+
+
+def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
             for ln in f:
@@ -239,6 +79,318 @@ def _cpu_model():
     except OSError:
         pass
     return None
+
+
+def threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def line(metric, value, unit, args, ms_per_step, dtype, config, **extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+class Timed:
+    """Run a plan for warmup + steps, with wall clock and the library's HIP-event kernel timing."""
+
+    def __init__(self, torch, plan, run_kwargs, stream):
+        self.torch, self.plan, self.kw, self.stream = torch, plan, run_kwargs, stream
+
+    def __call__(self, steps, warmup):
+        torch = self.torch
+        with torch.cuda.stream(self.stream):
+            for _ in range(warmup):
+                self.plan.execute(stream=self.stream, **self.kw)
+            torch.cuda.synchronize()
+            self.plan.set_timing(True)
+            t = time.perf_counter()
+            for _ in range(steps):
+                self.plan.execute(stream=self.stream, **self.kw)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t
+            item_ms, reduce_ms, count = self.plan.kernel_ms()
+            self.plan.set_timing(False)
+        return wall, item_ms, reduce_ms, count
+
+
+# ---- U-list workloads (headline: PVel) ---------------------------------------------------------------
+
+def ulist_bench(args, torch, S, kind, uker, points, cap, values_fn, kt, metric, workload_name, variants_order):
+    from stkfmm_amd.autotune import autotune
+
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    x = points()
+    tree = S.Tree(x, cap)
+    xs = x[tree.perm]
+    vals = values_fn(tree.n)
+    setup_s = time.perf_counter() - t0
+    d_x = torch.from_numpy(xs).to(dev)
+    d_v = [torch.from_numpy(v).to(dev) for v in vals]
+    out = torch.zeros((tree.n, kt), dtype=torch.float64, device=dev)
+    pairs = tree.pairs
+    stream = torch.cuda.Stream()
+    kw = dict(src_coord=d_x, src_value=d_v[0], trg_coord=d_x, trg_value=out)
+    if len(d_v) > 1:
+        kw["src_value2"] = d_v[1]
+
+    variants, results = {}, {}
+    order = variants_order if args.only is None else [args.only]
+    for name in order:
+        if name == "per-leaf" and args.no_per_leaf:
+            continue
+        prec = S.PREC.MIXED if name == "mixed" else S.PREC.FP64
+        if name == "per-leaf":
+            tune = dict(persistent=3)
+        else:
+            scratch = torch.zeros_like(out)
+            tune, _ = autotune(tree, uker, prec, dict(kw, trg_value=scratch, stream=stream))
+            del scratch
+        plan = S.Plan.from_tree(tree, uker, prec, tune=tune)
+        steps = args.steps if name != "per-leaf" else max(1, min(args.steps, 3))
+        warm = args.warmup if name != "per-leaf" else 1
+        wall, item_ms, reduce_ms, count = Timed(torch, plan, kw, stream)(steps, warm)
+        res = torch.zeros_like(out)
+        plan.execute(stream=stream, **dict(kw, trg_value=res))
+        torch.cuda.synchronize()
+        st = plan.stats()
+        assert st["pairs"] == pairs
+        variants[name] = dict(ms_per_step=1e3 * wall / steps, gpair_s=pairs * steps / wall / 1e9, item_ms=item_ms,
+                              reduce_ms=reduce_ms, timed_launches=count, n_items=st["n_items"],
+                              n_tiles=st["n_tiles"], grid=st["grid"], tune=tune)
+        results[name] = res
+        print(f"# {workload_name} {name}: {variants[name]['gpair_s']:.1f} Gpair/s, "
+              f"{variants[name]['ms_per_step']:.3f} ms/step, item kernel {item_ms:.3f} ms",
+              file=sys.stderr, flush=True)
+    head = order[0]
+    hv = variants[head]
+    T = (hv["tune"] or {}).get("targets_per_lane", 2)
+    prec_tag = "MIXED" if head == "mixed" else "FP64"
+    rl = roofline(kind, "mixed" if head == "mixed" else "fp64", pairs, hv["item_ms"],
+                  f"p2p_items_kernel<{int(uker)},{prec_tag},T={T}>")
+    tfile = os.path.join(ROOT, "profiles", f"{workload_name}_{head}_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            tr = json.load(f)
+        rl["traffic"] = tr.get("hbm_bytes_per_launch")
+        rl["traffic_source"] = os.path.relpath(tfile, ROOT)
+
+    accuracy, cpu = None, None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        nth = threads()
+        sz = (tree.leaf_end - tree.leaf_begin).astype(np.int64)
+        csum = np.concatenate([[0], np.cumsum(tree.rng_end.astype(np.int64) - tree.rng_begin)])
+        nsrc = csum[tree.ulist_ptr[1:]] - csum[tree.ulist_ptr[:-1]]
+        cost = sz * nsrc
+        allv = np.concatenate(vals, axis=1) if len(vals) > 1 else vals[0]
+        order_l = np.random.default_rng(12345).permutation(tree.n_leaves)
+        # calibrate on ~1% of the pairs, then run the rest of the workload if it fits twice the
+        # budget, or a random sample of leaves sized to the budget
+        cum = np.cumsum(cost[order_l])
+        cal = order_l[: int(np.searchsorted(cum, 0.01 * pairs)) + 1]
+        ref = np.zeros((tree.n, kt))
+        t = time.perf_counter()
+        ref = O.ranges(uker, xs, allv, xs, tree.leaf_begin, tree.leaf_end, tree.ulist_ptr, tree.rng_begin,
+                       tree.rng_end, sel=cal, nthreads=nth, simd=True, trg_value=ref)
+        cal_s = time.perf_counter() - t
+        rate = float(cost[cal].sum()) / max(cal_s, 1e-9)
+        rest = order_l[len(cal):]
+        if rate * args.cpu_seconds * 2 >= pairs:
+            more = rest
+        else:
+            budget = rate * args.cpu_seconds - float(cost[cal].sum())
+            more = rest[: int(np.searchsorted(np.cumsum(cost[rest]), max(budget, 0.0))) + 1]
+        t = time.perf_counter()
+        ref = O.ranges(uker, xs, allv, xs, tree.leaf_begin, tree.leaf_end, tree.ulist_ptr, tree.rng_begin,
+                       tree.rng_end, sel=more, nthreads=nth, simd=True, trg_value=ref)
+        main_s = time.perf_counter() - t
+        sel = np.concatenate([cal, more])
+        sample_pairs = int(cost[sel].sum())
+        cpu_s = cal_s + main_s
+        rows = np.concatenate([np.arange(tree.leaf_begin[l], tree.leaf_end[l]) for l in sel])
+        whole = len(sel) == tree.n_leaves
+        cpu = {"value": sample_pairs / cpu_s / 1e9, "unit": "Gpair/s", "cores": nth, "kind": "port",
+               "sample": (f"{'all' if whole else len(sel)} of {tree.n_leaves} leaves, {rows.size} targets, "
+                          f"{sample_pairs:.3e} pairs, {cpu_s:.1f} s; FP64 C restatement of the reference uKerEval "
+                          "algebra, AVX2 lanes over targets, OpenMP over leaves (PVFMM SIMD P2P not buildable "
+                          "offline)"),
+               "cpu_model": cpu_model()}
+        accuracy = {}
+        for name, res in results.items():
+            l2, mx = O.max_error(res.cpu().numpy()[rows], ref[rows])
+            accuracy[name] = {"errorL2": l2, "errorMaxRel": mx, "targets_checked": int(rows.size)}
+
+    extra = dict(variants=variants, accuracy_vs_oracle=accuracy, roofline=rl, cpu_baseline=cpu)
+    if "per-leaf" in variants and head == "mixed":
+        extra["speedup_vs_fp64_per_leaf_launch"] = hv["gpair_s"] / variants["per-leaf"]["gpair_s"]
+    if "fp64" in variants and head == "mixed":
+        extra["speedup_vs_fp64_batched"] = hv["gpair_s"] / variants["fp64"]["gpair_s"]
+    config = {"workload": workload_name, "points": tree.n, "leaf_cap": cap, "leaves": tree.n_leaves,
+              "max_level": tree.max_level, "pairs": pairs, "precision": head, "setup_s": setup_s}
+    line(metric, hv["gpair_s"], "Gpair/s", args, hv["ms_per_step"],
+         "f64+f32 (mixed)" if head == "mixed" else "f64", config, **extra)
+
+
+# ---- other workloads ------------------------------------------------------------------------------
+
+def lapgrad_direct(args, torch, S, W):
+    dev = torch.device("cuda:0")
+    src, q, trg = W.lapgrad_direct()
+    d_src, d_q, d_trg = (torch.from_numpy(a).to(dev) for a in (src, q, trg))
+    out = torch.zeros((len(trg), 4), dtype=torch.float64, device=dev)
+    uker = S.get_uker(S.KERNEL.LapPGrad, S.PPKERNEL.SLS2T)
+    plan = S.Plan(uker, S.PREC.FP64, len(src), len(trg), [0], [len(trg)], None, [0, 1], [0], [len(src)])
+    stream = torch.cuda.Stream()
+    kw = dict(src_coord=d_src, src_value=d_q, trg_coord=d_trg, trg_value=out)
+    steps = max(args.steps, 50)
+    wall, item_ms, reduce_ms, count = Timed(torch, plan, kw, stream)(steps, args.warmup)
+    pairs = len(src) * len(trg)
+    # the host-pointer entry (STKFMM::evaluateKernel drop-in): PCIe-inclusive
+    h_out = np.zeros((len(trg), 4))
+    S.evaluate_kernel(S.KERNEL.LapPGrad, 0, S.PPKERNEL.SLS2T, src, q, trg, h_out)
+    t = time.perf_counter()
+    for _ in range(5):
+        S.evaluate_kernel(S.KERNEL.LapPGrad, 0, S.PPKERNEL.SLS2T, src, q, trg, h_out)
+    host_s = (time.perf_counter() - t) / 5
+    cpu = acc = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        nth = threads()
+        t = time.perf_counter()
+        ref = O.direct(2, src, q, trg, nthreads=nth, chunked=True)
+        chunked_s = time.perf_counter() - t
+        t = time.perf_counter()
+        O.direct(2, src, q, trg, nthreads=nth)
+        all_s = time.perf_counter() - t
+        res = torch.zeros_like(out)
+        plan.execute(**dict(kw, trg_value=res))
+        torch.cuda.synchronize()
+        l2, mx = O.max_error(res.cpu().numpy(), ref)
+        acc = {"fp64": {"errorL2": l2, "errorMaxRel": mx}}
+        cpu = {"value": pairs / chunked_s / 1e9, "unit": "Gpair/s", "cores": nth, "kind": "port",
+               "sample": (f"whole workload; the reference's driver of FMMData.cpp:252-283 (4000-target chunks, "
+                          f"static(1): {len(trg) // 4000 + 1} busy threads) {chunked_s:.2f} s; all-core "
+                          f"target split {all_s:.2f} s = {pairs / all_s / 1e9:.3f} Gpair/s; scalar FP64 C "
+                          "restatement (PVFMM SIMD P2P not buildable offline)"),
+               "cpu_model": cpu_model()}
+    line("Gpair-interactions/s Laplace PGrad direct sum (evaluateKernel), FP64", pairs * steps / wall / 1e9,
+         "Gpair/s", args, 1e3 * wall / steps, "f64",
+         {"workload": "lapgrad-direct-8192x8192", "pairs": pairs, "precision": "fp64"},
+         host_api_pcie_inclusive_gpair_s=pairs / host_s / 1e9,
+         roofline=roofline("lapgrad", "fp64", pairs, item_ms, f"p2p_items_kernel<{int(uker)},FP64,T=2>"),
+         cpu_baseline=cpu, accuracy_vs_oracle=acc, kernel_ms={"item": item_ms, "reduce": reduce_ms})
+
+
+def pvel_adjoint(args, torch, S, W):
+    from stkfmm_amd.autograd import p2p
+    dev = torch.device("cuda:0")
+    src, val, trg, w = W.adjoint_pvel()
+    d_src, d_val, d_trg, d_w = (torch.from_numpy(a).to(dev) for a in (src, val, trg, w))
+    pairs = len(src) * len(trg)
+    uk = S.UKER.STOKES_PVEL
+    out = torch.zeros((len(trg), 4), dtype=torch.float64, device=dev)
+    for _ in range(args.warmup):
+        S.uker_device(uk, d_src, d_val, d_trg, out)
+        S.vjp_device(uk, d_src, d_val, d_trg, d_w)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        S.uker_device(uk, d_src, d_val, d_trg, out)
+    torch.cuda.synchronize()
+    fwd = (time.perf_counter() - t) / args.steps
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        S.vjp_device(uk, d_src, d_val, d_trg, d_w)
+    torch.cuda.synchronize()
+    adj = (time.perf_counter() - t) / args.steps
+    a = d_src.clone().requires_grad_(True)
+    b = d_val.clone().requires_grad_(True)
+    c = d_trg.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    (p2p(uk, a, b, c) * d_w).sum().backward()
+    torch.cuda.synchronize()
+    ag = time.perf_counter() - t
+    line("Gpair-interactions/s Stokes PVel forward + adjoint (direct sum, FP64)", 2 * pairs / (fwd + adj) / 1e9,
+         "Gpair/s", args, 1e3 * (fwd + adj), "f64",
+         {"workload": "pvel-adjoint-65536x65536", "pairs": pairs, "precision": "fp64",
+          "value_counts": "forward pairs + adjoint pairs"},
+         forward_ms=1e3 * fwd, adjoint_ms=1e3 * adj, autograd_fwd_bwd_ms=1e3 * ag,
+         forward_gpair_s=pairs / fwd / 1e9, adjoint_gpair_s=pairs / adj / 1e9)
+
+
+def stokeslet_instances(args, torch, S, W):
+    dev = torch.device("cuda:0")
+    sizes, offsets, coord, f = W.stokeslet_instances()
+    n = int(offsets[-1])
+    pairs = int(np.sum(sizes.astype(np.int64) ** 2))
+    d_c, d_f = torch.from_numpy(coord).to(dev), torch.from_numpy(f).to(dev)
+    res = {}
+    for prec in (S.PREC.MIXED, S.PREC.FP64):
+        gb, ge = offsets[:-1], offsets[1:]
+        plan = S.Plan(S.UKER.STOKES_VEL, prec, n, n, gb, ge, None, np.arange(len(sizes) + 1), gb, ge)
+        out = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+        stream = torch.cuda.Stream()
+        torch.cuda.synchronize()
+        plan.capture_graph(d_c, d_f, d_c, out, stream=stream)
+        for _ in range(args.warmup):
+            plan.launch_graph(stream=stream)
+        stream.synchronize()
+        lat = []
+        for _ in range(args.steps):
+            t = time.perf_counter()
+            plan.launch_graph(stream=stream)
+            stream.synchronize()
+            lat.append(time.perf_counter() - t)
+        res[prec.name.lower()] = dict(gpair_s=pairs / float(np.mean(lat)) / 1e9,
+                                      batch_latency_ms=1e3 * float(np.median(lat)),
+                                      batch_latency_min_ms=1e3 * float(np.min(lat)), n_items=plan.stats()["n_items"])
+    m = res["mixed"]
+    line("Gpair-interactions/s 512 Stokeslet direct sums in one HIP-graph replay (aggregate)", m["gpair_s"],
+         "Gpair/s", args, m["batch_latency_ms"], "f64+f32 (mixed)",
+         {"workload": "stokeslet-512-instances", "instances": len(sizes), "points": n, "pairs": pairs,
+          "precision": "mixed"}, variants=res)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(json.dumps({"error": "this build measures one GPU only: the P2P path runs on a single MI355X "
+                                   "(multi-GPU sharding is a design note, not a measured mode)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+
+    import stkfmm_amd as S
+    from stkfmm_amd import workloads as W
+
+    if args.copy_calibration:
+        n = 1 << 27  # 1 GiB of doubles
+        a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8}))
+        return
+    if args.workload == "pvel-ulist":
+        ulist_bench(args, torch, S, "pvel_fused", S.UKER.STOKES_PVEL_FUSED, W.pvel_points, 1000, W.pvel_values, 4,
+                    "Gpair-interactions/s StokesPVel P2P: FP64 vs mixed at ≤1e-6 rel err",
+                    "stokes-pvel-ulist-1M-clustered-leafcap1000", ["mixed", "fp64", "per-leaf"])
+    elif args.workload == "lapgrad-direct":
+        lapgrad_direct(args, torch, S, W)
+    elif args.workload == "rpy-ulist":
+        ulist_bench(args, torch, S, "rpy", S.UKER.RPY_ULAPU, W.rpy_points, 2000, lambda n: (W.rpy_values(n),), 6,
+                    "Gpair-interactions/s RPY P2P over an adaptive-octree U-list, mixed (autotuned) vs FP64",
+                    "rpy-ulist-4M-lognormal-leafcap2000", ["mixed", "fp64"])
+    elif args.workload == "pvel-adjoint":
+        pvel_adjoint(args, torch, S, W)
+    else:
+        stokeslet_instances(args, torch, S, W)
 
 
 if __name__ == "__main__":

@@ -180,6 +180,8 @@ typedef struct stkfmm_tune {
     int waves_per_cu;     /* persistent grid = CUs * waves_per_cu (capped by occupancy) */
     int split_tiles;      /* 0: a tile with fewer targets than lanes*T spreads each target over
                              several lanes (default); -1: one lane per target always */
+    int broadcast;        /* 0: sources staged through LDS per tile (default); 1: per-source records
+                             staged in HBM once per execute, read with scalar loads */
 } stkfmm_tune;
 
 typedef struct stkfmm_plan stkfmm_plan;

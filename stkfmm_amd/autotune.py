@@ -1,9 +1,9 @@
 """Autotuner for batched P2P plans.
 
 For a work list (an octree U-list or a batch of direct sums) and a micro-kernel/precision pair it
-times candidate tunings of ``stkfmm_tune`` -- targets per lane (1/2/4), sources per work item and
-the scheduling mode -- on the real data, with HIP events on the execution stream, and keeps the
-fastest.  Results are cached in JSON under a key made of the GPU architecture, micro-kernel,
+times candidate tunings of ``stkfmm_tune`` -- targets per lane (1/2/4), sources per work item, LDS or
+scalar-broadcast source staging and the scheduling mode -- on the real data, with HIP events on the
+execution stream, and keeps the fastest.  Results are cached in JSON under a key made of the GPU architecture, micro-kernel,
 precision and a coarse signature of the work list (target count, pair count and group count,
 bucketed by powers of two), so a later plan of the same class reuses the choice without timing.
 
@@ -21,8 +21,12 @@ from . import Plan, device_info
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULTS_PATH = os.path.join(HERE, "tuned_defaults.json")
 
-CANDIDATES = [dict(targets_per_lane=t, max_src_per_item=m, persistent=p)
-              for t, m, p in itertools.product((1, 2, 4), (1024, 2048, 4096), (1, 2))]
+# LDS-staged tiles over the full grid; scalar-broadcast records (wins for FP64 on MI355X) over the
+# sizes where it is competitive
+CANDIDATES = ([dict(targets_per_lane=t, max_src_per_item=m, persistent=p, broadcast=0)
+               for t, m, p in itertools.product((1, 2, 4), (1024, 2048, 4096), (1, 2))]
+              + [dict(targets_per_lane=t, max_src_per_item=m, persistent=p, broadcast=1)
+                 for t, m, p in itertools.product((1, 2), (1024, 2048), (1, 2))])
 
 
 def cache_path():

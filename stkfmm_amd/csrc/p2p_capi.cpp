@@ -187,6 +187,10 @@ struct stkfmm_plan {
     std::vector<hipEvent_t> ev;
     long long n_timed = 0;
     std::vector<int> group_items; // mode 3: item index ranges per launched group (host)
+    // scalar-broadcast variant: staged source records, one copy per level in [level_min, level_max]
+    int broadcast = 0;
+    int level_min = 0, n_copies = 0, rec_words = 0;
+    double *d_recs = nullptr;
 };
 
 static void plan_free(stkfmm_plan *p) {
@@ -203,6 +207,7 @@ static void plan_free(stkfmm_plan *p) {
     (void)hipFree(p->d_tiles);
     (void)hipFree(p->d_partial);
     (void)hipFree(p->d_counter);
+    (void)hipFree(p->d_recs);
     delete p;
 }
 
@@ -232,7 +237,7 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
     if (n_groups > 0 && (!grp_trg_begin || !grp_trg_end || !grp_rng_ptr))
         return set_error(STKFMM_ERR_ARG, "group arrays are NULL");
 
-    stkfmm_tune tn{0, 0, 0, 0, 0};
+    stkfmm_tune tn{0, 0, 0, 0, 0, 0};
     if (tune)
         tn = *tune;
     const int T = tn.targets_per_lane ? tn.targets_per_lane : 2;
@@ -241,6 +246,9 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
     const int persistent = tn.persistent ? tn.persistent : 1;
     if (persistent < 1 || persistent > 3)
         return set_error(STKFMM_ERR_ARG, "persistent must be 1, 2 or 3 (got %d)", persistent);
+    const int broadcast = tn.broadcast;
+    if (broadcast != 0 && broadcast != 1)
+        return set_error(STKFMM_ERR_ARG, "broadcast must be 0 (LDS) or 1 (scalar) (got %d)", broadcast);
 
     // validate the work list
     long long total_cost = 0; // sum over tiles of the tile's source count
@@ -270,7 +278,10 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
     int rc = device_cu_count(&n_cu);
     if (rc)
         return rc;
-    HIP_TRY(items_occupancy(uk, precision, T, &bpc), "occupancy query");
+    if (broadcast)
+        HIP_TRY(items_sgpr_occupancy(uk, precision, T, &bpc), "occupancy query");
+    else
+        HIP_TRY(items_occupancy(uk, precision, T, &bpc), "occupancy query");
     if (bpc < 1)
         bpc = 1;
     const int concurrent = n_cu * bpc;
@@ -366,6 +377,20 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
     p->grid = (persistent == 1) ? (int)std::min<long long>(p->n_items, (long long)n_cu * wpc) : p->n_items;
     if (persistent == 3)
         p->group_items = group_items;
+    p->broadcast = broadcast;
+    if (broadcast) {
+        int lmin = 0, lmax = 0;
+        if (precision != STKFMM_PREC_FP64 && grp_level && n_groups > 0) {
+            lmin = lmax = grp_level[0];
+            for (int g = 1; g < n_groups; ++g) {
+                lmin = std::min(lmin, grp_level[g]);
+                lmax = std::max(lmax, grp_level[g]);
+            }
+        }
+        p->level_min = lmin;
+        p->n_copies = lmax - lmin + 1;
+        p->rec_words = record_words(uk, precision);
+    }
     p->pairs = pairs;
     (void)hipGetDevice(&p->device);
 
@@ -381,6 +406,8 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
         e = hipMalloc((void **)&p->d_partial, sizeof(double) * partial_elems);
     if (e == hipSuccess)
         e = hipMalloc((void **)&p->d_counter, sizeof(int));
+    if (e == hipSuccess && p->broadcast && n_src > 0)
+        e = hipMalloc((void **)&p->d_recs, sizeof(double) * (size_t)p->n_copies * n_src * p->rec_words);
     if (e != hipSuccess) {
         plan_free(p);
         return hip_error(e, "plan workspace");
@@ -410,6 +437,7 @@ static int plan_enqueue(stkfmm_plan *p, const stkfmm_array *sc, const stkfmm_arr
     a.n_items = p->n_items;
     a.counter = (p->persistent == 1) ? p->d_counter : nullptr;
     a.partial = p->d_partial;
+    a.n_src_rec = p->n_src;
     ReduceArgs ra;
     ra.tiles = p->d_tiles;
     ra.n_tiles = p->n_tiles;
@@ -427,16 +455,24 @@ static int plan_enqueue(stkfmm_plan *p, const stkfmm_array *sc, const stkfmm_arr
     }
     if (p->persistent == 1)
         HIP_TRY(launch_fill_int(p->d_counter, 0, s), "counter reset launch");
+    if (p->broadcast) {
+        StageArgs sa{a.sc, a.v1, a.v2, p->n_src, p->level_min, p->d_recs};
+        HIP_TRY(launch_stage_records(p->uk, p->prec, sa, p->n_copies, s), "stage records launch");
+    }
+    auto launch = [&](int grid, const ItemArgs &ia) {
+        return p->broadcast ? launch_items_sgpr(p->uk, p->prec, p->T, grid, ia, p->d_recs, p->level_min, s)
+                            : launch_items(p->uk, p->prec, p->T, grid, ia, s);
+    };
     if (p->persistent == 3) {
         // one launch per target group, in work-list order (the per-leaf-launch scheme)
         for (size_t g = 0; g + 1 < p->group_items.size(); ++g) {
             ItemArgs ag = a;
             ag.items = p->d_items + p->group_items[g];
             ag.n_items = p->group_items[g + 1] - p->group_items[g];
-            HIP_TRY(launch_items(p->uk, p->prec, p->T, ag.n_items, ag, s), "p2p items launch");
+            HIP_TRY(launch(ag.n_items, ag), "p2p items launch");
         }
     } else {
-        HIP_TRY(launch_items(p->uk, p->prec, p->T, p->grid, a, s), "p2p items launch");
+        HIP_TRY(launch(p->grid, a), "p2p items launch");
     }
     if (timed)
         HIP_TRY(hipEventRecord(ev[1], s), "event record");

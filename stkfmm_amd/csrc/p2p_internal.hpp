@@ -59,6 +59,15 @@ struct ItemArgs {
     int n_items;
     int *counter;         // persistent mode: next item to take; null: one workgroup per item
     double *partial;      // [slot][KT][64*T]
+    long long n_src_rec;  // scalar-broadcast variant: sources per staged record copy
+};
+
+// scalar-broadcast variant: per-source records staged once per execute (one copy per level)
+struct StageArgs {
+    DArr sc, v1, v2;
+    long long n_src;
+    int level_min;        // copy c holds sources scaled by 2^(level_min + c) (mixed mode)
+    double *rec;          // [copy][n_src][words]
 };
 
 struct ReduceArgs {
@@ -98,6 +107,12 @@ hipError_t launch_items(int uk, int prec, int T, int grid, const ItemArgs &a, hi
 hipError_t launch_reduce(const ReduceArgs &a, hipStream_t s);
 hipError_t launch_fill_int(int *p, int value, hipStream_t s);
 hipError_t items_occupancy(int uk, int prec, int T, int *blocks_per_cu);
+// scalar-broadcast variant
+int record_words(int uk, int prec);
+hipError_t launch_stage_records(int uk, int prec, const StageArgs &a, int n_copies, hipStream_t s);
+hipError_t launch_items_sgpr(int uk, int prec, int T, int grid, const ItemArgs &a, const double *recs,
+                             int level_min, hipStream_t s);
+hipError_t items_sgpr_occupancy(int uk, int prec, int T, int *blocks_per_cu);
 // chunk counts: the target pass splits the sources into nchunk_t chunks, the source pass splits the
 // targets into nchunk_s chunks; partial must hold max(nchunk_t*3*n_trg, nchunk_s*(3+KS)*n_src)
 hipError_t launch_vjp(int uk, int prec, const VjpArgs &a, double *partial, int nchunk_t, int nchunk_s,

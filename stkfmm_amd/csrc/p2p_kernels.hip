@@ -21,6 +21,7 @@
 
 #include "p2p_internal.hpp"
 #include "ukernels.hpp"
+#include "p2p_sgpr.hpp"
 #include "vjp_kernels.hpp"
 
 namespace stkp2p {
@@ -311,6 +312,100 @@ hipError_t launch_items(int uk, int prec, int T, int grid, const ItemArgs &a, hi
 hipError_t items_occupancy(int uk, int prec, int T, int *blocks_per_cu) {
 #define CALL(U) occ_p<U>(prec, T, blocks_per_cu)
     STKP2P_UK_SWITCH(CALL)
+#undef CALL
+}
+
+// ---- scalar-broadcast variant dispatch ----------------------------------------------------------
+
+template <int UK, int PREC, int T>
+static hipError_t launch_sgpr_one(int grid, const ItemArgs &a, const double *recs, int level_min, hipStream_t s) {
+    hipLaunchKernelGGL((p2p_items_sgpr_kernel<UK, PREC, T>), dim3(grid), dim3(kWave), 0, s, a, recs, level_min);
+    return hipGetLastError();
+}
+
+template <int UK, int PREC, int T>
+static hipError_t occ_sgpr_one(int *blocks) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, p2p_items_sgpr_kernel<UK, PREC, T>, kWave, 0);
+}
+
+template <int UK, int PREC>
+static hipError_t launch_sgpr_t(int T, int grid, const ItemArgs &a, const double *recs, int lmin, hipStream_t s) {
+    switch (T) {
+    case 1: return launch_sgpr_one<UK, PREC, 1>(grid, a, recs, lmin, s);
+    case 2: return launch_sgpr_one<UK, PREC, 2>(grid, a, recs, lmin, s);
+    case 4: return launch_sgpr_one<UK, PREC, 4>(grid, a, recs, lmin, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int UK, int PREC>
+static hipError_t occ_sgpr_t(int T, int *blocks) {
+    switch (T) {
+    case 1: return occ_sgpr_one<UK, PREC, 1>(blocks);
+    case 2: return occ_sgpr_one<UK, PREC, 2>(blocks);
+    case 4: return occ_sgpr_one<UK, PREC, 4>(blocks);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int UK>
+static hipError_t stage_p(int prec, const StageArgs &a, int n_copies, hipStream_t s) {
+    const dim3 grid((unsigned)((a.n_src + 255) / 256), (unsigned)n_copies);
+    if (prec == 0)
+        hipLaunchKernelGGL((stage_records_kernel<UK, 0>), grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((stage_records_kernel<UK, 1>), grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int UK>
+static int words_p(int prec) {
+    using K = typename UkerOf<UK>::type;
+    return prec == 0 ? RecLayout<K, 0>::WORDS : RecLayout<K, 1>::WORDS;
+}
+
+hipError_t launch_items_sgpr(int uk, int prec, int T, int grid, const ItemArgs &a, const double *recs,
+                             int level_min, hipStream_t s) {
+    if (grid <= 0)
+        return hipSuccess;
+#define CALL(U) (prec == 0 ? launch_sgpr_t<U, 0>(T, grid, a, recs, level_min, s) : launch_sgpr_t<U, 1>(T, grid, a, recs, level_min, s))
+    STKP2P_UK_SWITCH(CALL)
+#undef CALL
+}
+
+hipError_t items_sgpr_occupancy(int uk, int prec, int T, int *blocks_per_cu) {
+#define CALL(U) (prec == 0 ? occ_sgpr_t<U, 0>(T, blocks_per_cu) : occ_sgpr_t<U, 1>(T, blocks_per_cu))
+    STKP2P_UK_SWITCH(CALL)
+#undef CALL
+}
+
+hipError_t launch_stage_records(int uk, int prec, const StageArgs &a, int n_copies, hipStream_t s) {
+    if (a.n_src <= 0 || n_copies <= 0)
+        return hipSuccess;
+#define CALL(U) stage_p<U>(prec, a, n_copies, s)
+    STKP2P_UK_SWITCH(CALL)
+#undef CALL
+}
+
+int record_words(int uk, int prec) {
+#define CALL(U) words_p<U>(prec)
+    switch (uk) {
+    case 0: return CALL(0);
+    case 1: return CALL(1);
+    case 2: return CALL(2);
+    case 3: return CALL(3);
+    case 4: return CALL(4);
+    case 5: return CALL(5);
+    case 6: return CALL(6);
+    case 7: return CALL(7);
+    case 8: return CALL(8);
+    case 9: return CALL(9);
+    case 10: return CALL(10);
+    case 11: return CALL(11);
+    case 12: return CALL(12);
+    case 13: return CALL(13);
+    default: return -1;
+    }
 #undef CALL
 }
 

@@ -118,3 +118,15 @@ def test_ker_t_dof_must_be_one():
     with pytest.raises(S.Unsupported):
         S.ker_eval(S.UKER.STOKES_PVEL, rng.random((4, 3)), np.zeros((4, 4)), rng.random((3, 3)),
                    np.zeros((3, 4)), dof=2)
+
+
+@pytest.mark.parametrize("flag,msg", [("--inactive-kernel", "Error: no such FMMData exists for kernel rpy_ulapu"),
+                                      ("--unknown-kernel", "Error: Kernel not found.")])
+def test_cxx_interface_error_paths_exit_like_the_reference(flag, msg):
+    """include/stkfmm_p2p.hpp: inactive and unknown kernels print and exit(1) (STKFMM.cpp:53-55,92-95)."""
+    import subprocess
+    driver = os.path.join(ROOT, "tests", "cxx", "direct_driver")
+    if not os.path.exists(driver):
+        pytest.skip("C++ driver not built")
+    r = subprocess.run([driver, flag], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and msg in r.stdout, (r.returncode, r.stdout, r.stderr)

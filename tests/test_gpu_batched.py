@@ -45,14 +45,15 @@ def leaf_rows(tr, sel):
     return np.concatenate([np.arange(tr.leaf_begin[l], tr.leaf_end[l]) for l in sel])
 
 
+@pytest.mark.parametrize("bc", [0, 1], ids=["lds", "scalar"])
 @pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
 @pytest.mark.parametrize("uk", [13, 2, 11, 6, 9, 7, 8, 4])
-def test_small_tree_all_leaves(uk, prec):
-    say(f"small tree uker={uk} prec={prec}")
+def test_small_tree_all_leaves(uk, prec, bc):
+    say(f"small tree uker={uk} prec={prec} broadcast={bc}")
     g = np.random.default_rng(uk)
     x = W.wrap(np.concatenate([g.random((6000, 3)), 0.6 + 0.02 * g.normal(size=(6000, 3))]))
     tr, xs, v, dl = sorted_problem(x, 150, uk, 10 + uk)
-    out, plan = run_plan(tr, uk, prec, xs, v, dl)
+    out, plan = run_plan(tr, uk, prec, xs, v, dl, tune=dict(broadcast=bc))
     assert plan.stats()["pairs"] == tr.pairs
     ref = oracle_leaves(tr, uk, xs, v, dl, None)
     l2, mx = O.max_error(out, ref)
@@ -63,7 +64,10 @@ def test_small_tree_all_leaves(uk, prec):
 @pytest.mark.parametrize("tune", [dict(targets_per_lane=1), dict(targets_per_lane=2, persistent=2),
                                   dict(targets_per_lane=4, max_src_per_item=300),
                                   dict(targets_per_lane=2, max_src_per_item=64, waves_per_cu=1),
-                                  dict(targets_per_lane=2, split_tiles=-1), dict(targets_per_lane=4, split_tiles=0)])
+                                  dict(targets_per_lane=2, split_tiles=-1), dict(targets_per_lane=4, split_tiles=0),
+                                  dict(targets_per_lane=1, broadcast=1), dict(targets_per_lane=2, broadcast=1),
+                                  dict(targets_per_lane=4, broadcast=1, persistent=2),
+                                  dict(targets_per_lane=2, broadcast=1, persistent=3)])
 def test_tuning_knobs_do_not_change_results(tune):
     say(f"tune {tune}")
     g = np.random.default_rng(7)
