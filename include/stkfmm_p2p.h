@@ -65,7 +65,18 @@ enum stkfmm_uker {
     STKFMM_UKER_RPY_ULAPU = 11,              /* rpy_ulapu               4 -> 6   RPYKernel.hpp:52-90 */
     STKFMM_UKER_STK_ULAPU = 12,              /* stk_ulapu               3 -> 6   RPYKernel.hpp:97-129 */
     STKFMM_UKER_STOKES_PVEL_FUSED = 13,      /* stokes_pvel + stokes_doublepvel at one point, 4+9 -> 4 */
-    STKFMM_UKER_COUNT = 14
+    STKFMM_UKER_STOKES_PVELGRAD = 14,        /* stokes_pvelgrad         4 -> 16  StokesSingleLayerKernel.hpp:54-111 */
+    STKFMM_UKER_STOKES_DOUBLEPVELGRAD = 15,  /* stokes_doublepvelgrad   9 -> 16  StokesDoubleLayerKernel.hpp:64-143 */
+    STKFMM_UKER_STOKES_PVELLAPLACIAN = 16,   /* stokes_pvellaplacian    4 -> 7   StokesSingleLayerKernel.hpp:118-155 */
+    STKFMM_UKER_STOKES_DOUBLELAPLACIAN = 17, /* stokes_doublelaplacian  9 -> 7   StokesDoubleLayerKernel.hpp:150-206 */
+    STKFMM_UKER_STOKES_TRACTION = 18,        /* stokes_traction         4 -> 9   StokesSingleLayerKernel.hpp:162-195 */
+    STKFMM_UKER_STOKES_DOUBLETRACTION = 19,  /* stokes_doubletraction   9 -> 9   StokesDoubleLayerKernel.hpp:213-273 */
+    STKFMM_UKER_STOKES_REGFTVELOMEGA = 20,   /* stokes_regftvelomega    7 -> 6   StokesRegSingleLayerKernel.hpp:96-151 */
+    STKFMM_UKER_STOKES_VELOMEGA = 21,        /* stokes_velomega         3 -> 6   StokesRegSingleLayerKernel.hpp:158-202 */
+    STKFMM_UKER_LAPLACE_QUADPGRADGRAD = 22,  /* laplace_quadpgradgrad   9 -> 10  LaplaceLayerKernel.hpp:234-312 */
+    STKFMM_UKER_LAPLACE_QUADP = 23,          /* laplace_quadp           9 -> 1   LaplaceLayerKernel.hpp:202-232 */
+    STKFMM_UKER_STOKES_REGFTVEL = 24,        /* stokes_regftvel         7 -> 3   StokesRegSingleLayerKernel.hpp:48-89 */
+    STKFMM_UKER_COUNT = 25
 };
 
 /* status codes */
@@ -149,6 +160,28 @@ void stkfmm_ker_rpy_ulapu(double *r_src, int src_cnt, double *v_src, int dof, do
                           double *v_trg, void *mem_mgr);
 void stkfmm_ker_stk_ulapu(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg, int trg_cnt,
                           double *v_trg, void *mem_mgr);
+void stkfmm_ker_stokes_pvelgrad(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg, int trg_cnt,
+                                double *v_trg, void *mem_mgr);
+void stkfmm_ker_stokes_doublepvelgrad(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg,
+                                      int trg_cnt, double *v_trg, void *mem_mgr);
+void stkfmm_ker_stokes_pvellaplacian(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg,
+                                     int trg_cnt, double *v_trg, void *mem_mgr);
+void stkfmm_ker_stokes_doublelaplacian(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg,
+                                       int trg_cnt, double *v_trg, void *mem_mgr);
+void stkfmm_ker_stokes_traction(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg, int trg_cnt,
+                                double *v_trg, void *mem_mgr);
+void stkfmm_ker_stokes_doubletraction(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg,
+                                      int trg_cnt, double *v_trg, void *mem_mgr);
+void stkfmm_ker_stokes_regftvelomega(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg,
+                                     int trg_cnt, double *v_trg, void *mem_mgr);
+void stkfmm_ker_stokes_velomega(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg, int trg_cnt,
+                                double *v_trg, void *mem_mgr);
+void stkfmm_ker_laplace_quadpgradgrad(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg,
+                                      int trg_cnt, double *v_trg, void *mem_mgr);
+void stkfmm_ker_laplace_quadp(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg, int trg_cnt,
+                              double *v_trg, void *mem_mgr);
+void stkfmm_ker_stokes_regftvel(double *r_src, int src_cnt, double *v_src, int dof, double *r_trg, int trg_cnt,
+                                double *v_trg, void *mem_mgr);
 
 /* generic form of the entries above, status-returning */
 int stkfmm_ker_eval(int uker, const double *r_src, int src_cnt, const double *v_src, int dof,

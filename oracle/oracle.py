@@ -18,7 +18,10 @@ REF_LIB_PATH = os.path.join(HERE, "_ref", "libstkfmm_simplekernel.so")
 UK = dict(
     LAPLACE_P=0, LAPLACE_DIPOLEP=1, LAPLACE_PGRAD=2, LAPLACE_DIPOLEPGRAD=3, LAPLACE_PGRADGRAD=4,
     LAPLACE_DIPOLEPGRADGRAD=5, STOKES_VEL=6, STOKES_PVEL=7, STOKES_DOUBLEPVEL=8, STOKES_REGVEL=9,
-    RPY_U=10, RPY_ULAPU=11, STK_ULAPU=12, STOKES_PVEL_FUSED=13,
+    RPY_U=10, RPY_ULAPU=11, STK_ULAPU=12, STOKES_PVEL_FUSED=13, STOKES_PVELGRAD=14,
+    STOKES_DOUBLEPVELGRAD=15, STOKES_PVELLAPLACIAN=16, STOKES_DOUBLELAPLACIAN=17, STOKES_TRACTION=18,
+    STOKES_DOUBLETRACTION=19, STOKES_REGFTVELOMEGA=20, STOKES_VELOMEGA=21, LAPLACE_QUADPGRADGRAD=22,
+    LAPLACE_QUADP=23, STOKES_REGFTVEL=24,
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)

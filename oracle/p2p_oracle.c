@@ -51,22 +51,38 @@ enum {
     UK_RPY_ULAPU = 11,              /* 4 -> 6  */
     UK_STK_ULAPU = 12,              /* 3 -> 6  */
     UK_STOKES_PVEL_FUSED = 13,      /* 4+9 -> 4: SL and DL at the same point, one pair */
-    UK_COUNT = 14
+    UK_STOKES_PVELGRAD = 14,        /* 4 -> 16 */
+    UK_STOKES_DOUBLEPVELGRAD = 15,  /* 9 -> 16 */
+    UK_STOKES_PVELLAPLACIAN = 16,   /* 4 -> 7  */
+    UK_STOKES_DOUBLELAPLACIAN = 17, /* 9 -> 7  */
+    UK_STOKES_TRACTION = 18,        /* 4 -> 9  */
+    UK_STOKES_DOUBLETRACTION = 19,  /* 9 -> 9  */
+    UK_STOKES_REGFTVELOMEGA = 20,   /* 7 -> 6  */
+    UK_STOKES_VELOMEGA = 21,        /* 3 -> 6  */
+    UK_LAPLACE_QUADPGRADGRAD = 22,  /* 9 -> 10 */
+    UK_LAPLACE_QUADP = 23,          /* 9 -> 1  */
+    UK_STOKES_REGFTVEL = 24,        /* 7 -> 3  */
+    UK_COUNT = 25
 };
 
-static const int uk_src_dim[UK_COUNT] = {1, 3, 1, 3, 1, 3, 3, 4, 9, 4, 4, 4, 3, 13};
-static const int uk_trg_dim[UK_COUNT] = {1, 1, 4, 4, 10, 10, 3, 4, 4, 3, 3, 6, 6, 4};
+static const int uk_src_dim[UK_COUNT] = {1, 3, 1, 3, 1, 3, 3, 4, 9, 4, 4, 4, 3, 13, 4, 9, 4, 9, 4, 9, 7, 3, 9, 9, 7};
+static const int uk_trg_dim[UK_COUNT] = {1, 1, 4, 4, 10, 10, 3, 4, 4, 3, 3, 6, 6, 4, 16, 16, 7, 7, 9, 9, 6, 6, 10, 1, 3};
 
 int oracle_uker_count(void) { return UK_COUNT; }
 int oracle_uker_src_dim(int uk) { return (uk >= 0 && uk < UK_COUNT) ? uk_src_dim[uk] : -1; }
 int oracle_uker_trg_dim(int uk) { return (uk >= 0 && uk < UK_COUNT) ? uk_trg_dim[uk] : -1; }
 
-/* ScaleFactor<Real>(): 1/(4 pi) for the Laplace family (LaplaceLayerKernel.hpp:44-46),
- * 1/(8 pi) for the Stokes family (StokesSingleLayerKernel.hpp:30-32, RPYKernel.hpp:20-22, ...). */
+/* ScaleFactor<Real>(): 1/(4 pi) for the Laplace family (LaplaceLayerKernel.hpp:45,206,238), -3/(4 pi) for stokes_traction (StokesSingleLayerKernel.hpp:166), -3/(8 pi) for
+ * stokes_doubletraction (StokesDoubleLayerKernel.hpp:217), 1/(8 pi) for the rest of the Stokes
+ * family (StokesSingleLayerKernel.hpp:30-32, RPYKernel.hpp:20-22, ...). */
 double oracle_uker_scale(int uk) {
     const double pi = 3.141592653589793238462643383279;
-    if (uk <= UK_LAPLACE_DIPOLEPGRADGRAD)
+    if (uk <= UK_LAPLACE_DIPOLEPGRADGRAD || uk == UK_LAPLACE_QUADPGRADGRAD || uk == UK_LAPLACE_QUADP)
         return 1.0 / (4.0 * pi);
+    if (uk == UK_STOKES_TRACTION)
+        return -3.0 / (4.0 * pi);
+    if (uk == UK_STOKES_DOUBLETRACTION)
+        return -3.0 / (8.0 * pi);
     return 1.0 / (8.0 * pi);
 }
 
@@ -310,12 +326,390 @@ static inline void uk_stokes_pvel_fused(double *u, const double *r, const double
     uk_stokes_doublepvel(u, r, f + 4);
 }
 
+/* ---- the rest of the Stokes layer family (StokesSingleLayerKernel.hpp,
+ *      StokesDoubleLayerKernel.hpp) ------------------------------------------------------------ */
+
+/* stokes_pvelgrad::uKerEval, StokesSingleLayerKernel.hpp:61-110; source [fx,fy,fz,TrD] ->
+ * [p, u(3), grad p(3), grad u(9) row-major: du_i/dx_j at 7 + 3i + j].  The reference subtracts and
+ * re-adds TrD to its running coefficient (:78,84,98); that sequence of roundings is kept. */
+static inline void uk_stokes_pvelgrad(double *u, const double *r, const double *f) {
+    const double dx = r[0], dy = r[1], dz = r[2];
+    const double fx = f[0], fy = f[1], fz = f[2], tr = f[3];
+    double r2 = dx * dx + dy * dy + dz * dz;
+    double rinv = rsqrt_masked(r2);
+    double rinv3 = rinv * rinv * rinv;
+    double rinv5 = rinv3 * rinv * rinv;
+    const double two = 2.0, nthree = -3.0;
+    double cc = fx * dx + fy * dy + fz * dz;
+    u[0] += two * rinv3 * cc;
+    cc -= tr;
+    u[1] += rinv3 * (r2 * fx + cc * dx);
+    u[2] += rinv3 * (r2 * fy + cc * dy);
+    u[3] += rinv3 * (r2 * fz + cc * dz);
+    cc += tr;
+    u[4] += two * (r2 * fx + nthree * dx * cc) * rinv5;
+    u[5] += two * (r2 * fy + nthree * dy * cc) * rinv5;
+    u[6] += two * (r2 * fz + nthree * dz * cc) * rinv5;
+    double qxx = r2 + nthree * dx * dx;
+    double qxy = nthree * dx * dy;
+    double qxz = nthree * dx * dz;
+    double qyy = r2 + nthree * dy * dy;
+    double qyz = nthree * dy * dz;
+    double qzz = r2 + nthree * dz * dz;
+    cc -= tr;
+    u[7] += qxx * cc * rinv5;
+    u[8] += (qxy * cc + r2 * (dx * fy - dy * fx)) * rinv5;
+    u[9] += (qxz * cc + r2 * (dx * fz - dz * fx)) * rinv5;
+    u[10] += (qxy * cc + r2 * (dy * fx - dx * fy)) * rinv5;
+    u[11] += qyy * cc * rinv5;
+    u[12] += (qyz * cc + r2 * (dy * fz - dz * fy)) * rinv5;
+    u[13] += (qxz * cc + r2 * (dz * fx - dx * fz)) * rinv5;
+    u[14] += (qyz * cc + r2 * (dz * fy - dy * fz)) * rinv5;
+    u[15] += qzz * cc * rinv5;
+}
+
+/* stokes_pvellaplacian::uKerEval, StokesSingleLayerKernel.hpp:125-154; [fx,fy,fz,TrD] ->
+ * [p, u(3), lap u(3)] */
+static inline void uk_stokes_pvellaplacian(double *u, const double *r, const double *f) {
+    const double dx = r[0], dy = r[1], dz = r[2];
+    const double fx = f[0], fy = f[1], fz = f[2], tr = f[3];
+    double r2 = dx * dx + dy * dy + dz * dz;
+    double rinv = rsqrt_masked(r2);
+    double rinv3 = rinv * rinv * rinv;
+    double rinv5 = rinv3 * rinv * rinv;
+    const double two = 2.0, nthree = -3.0;
+    double cc = fx * dx + fy * dy + fz * dz;
+    u[0] += two * rinv3 * cc;
+    cc -= tr;
+    u[1] += rinv3 * (r2 * fx + cc * dx);
+    u[2] += rinv3 * (r2 * fy + cc * dy);
+    u[3] += rinv3 * (r2 * fz + cc * dz);
+    cc = nthree * (cc + tr);
+    u[4] += two * (fx * r2 + cc * dx) * rinv5;
+    u[5] += two * (fy * r2 + cc * dy) * rinv5;
+    u[6] += two * (fz * r2 + cc * dz) * rinv5;
+}
+
+/* stokes_traction::uKerEval, StokesSingleLayerKernel.hpp:169-194 (ScaleFactor -3/(4 pi), :166);
+ * [fx,fy,fz,TrD] -> stress row-major (9) */
+static inline void uk_stokes_traction(double *u, const double *r, const double *f) {
+    const double dx = r[0], dy = r[1], dz = r[2];
+    const double fx = f[0], fy = f[1], fz = f[2], tr = f[3];
+    double r2 = dx * dx + dy * dy + dz * dz;
+    double rinv = rsqrt_masked(r2);
+    double rinv3 = rinv * rinv * rinv;
+    double rinv5 = rinv3 * rinv * rinv;
+    const double invthree = 0.3333333333333333333;
+    double cc = (fx * dx + fy * dy + fz * dz - tr) * rinv5;
+    double diag = tr * r2 * rinv5 * invthree;
+    u[0] += dx * dx * cc + diag;
+    u[1] += dx * dy * cc;
+    u[2] += dx * dz * cc;
+    u[3] += dy * dx * cc;
+    u[4] += dy * dy * cc + diag;
+    u[5] += dy * dz * cc;
+    u[6] += dz * dx * cc;
+    u[7] += dz * dy * cc;
+    u[8] += dz * dz * cc + diag;
+}
+
+/* common sub-expressions of the double-layer gradient kernels (StokesDoubleLayerKernel.hpp:88-103,
+ * 174-189, 239-250): Q = r.D.r, its -3 and 5 multiples, trace, and D r, D^T r */
+struct dl_terms {
+    double cc, ccn3, cc5, trace, rksxk, rksyk, rkszk, rkskx, rksky, rkskz;
+};
+static inline struct dl_terms dl_common(const double *r, const double *f) {
+    const double sxx = f[0], sxy = f[1], sxz = f[2];
+    const double syx = f[3], syy = f[4], syz = f[5];
+    const double szx = f[6], szy = f[7], szz = f[8];
+    const double dx = r[0], dy = r[1], dz = r[2];
+    struct dl_terms t;
+    t.cc = sxx * dx * dx + syy * dy * dy + szz * dz * dz;
+    t.cc += (sxy + syx) * dx * dy;
+    t.cc += (sxz + szx) * dx * dz;
+    t.cc += (syz + szy) * dy * dz;
+    t.ccn3 = (-3.0) * t.cc;
+    t.cc5 = (5.0) * t.cc;
+    t.trace = sxx + syy + szz;
+    t.rksxk = dx * sxx + dy * sxy + dz * sxz;
+    t.rksyk = dx * syx + dy * syy + dz * syz;
+    t.rkszk = dx * szx + dy * szy + dz * szz;
+    t.rkskx = dx * sxx + dy * syx + dz * szx;
+    t.rksky = dx * sxy + dy * syy + dz * szy;
+    t.rkskz = dx * sxz + dy * syz + dz * szz;
+    return t;
+}
+
+/* stokes_doublepvelgrad::uKerEval, StokesDoubleLayerKernel.hpp:71-142; D row-major (9) -> 16 */
+static inline void uk_stokes_doublepvelgrad(double *u, const double *r, const double *f) {
+    const double sxx = f[0], sxy = f[1], sxz = f[2];
+    const double syx = f[3], syy = f[4], syz = f[5];
+    const double szx = f[6], szy = f[7], szz = f[8];
+    const double dx = r[0], dy = r[1], dz = r[2];
+    double r2 = dx * dx + dy * dy + dz * dz;
+    double rinv = rsqrt_masked(r2);
+    double rinv2 = rinv * rinv;
+    double rinv3 = rinv * rinv2;
+    double rinv5 = rinv3 * rinv2;
+    double rinv7 = rinv5 * rinv2;
+    const double two = 2.0, three = 3.0, five = 5.0;
+    struct dl_terms t = dl_common(r, f);
+    u[0] += two * (t.ccn3 + r2 * t.trace) * rinv5;
+    u[1] += rinv5 * dx * t.ccn3;
+    u[2] += rinv5 * dy * t.ccn3;
+    u[3] += rinv5 * dz * t.ccn3;
+    rinv7 *= -three;
+    u[4] -= two * rinv7 * (dx * t.cc5 - r2 * ((t.rksxk + t.rkskx) + dx * t.trace));
+    u[5] -= two * rinv7 * (dy * t.cc5 - r2 * ((t.rksyk + t.rksky) + dy * t.trace));
+    u[6] -= two * rinv7 * (dz * t.cc5 - r2 * ((t.rkszk + t.rkskz) + dz * t.trace));
+    double ccn1 = -t.cc;
+    double dcFd0 = -two * dx * sxx - dy * (sxy + syx) - dz * (sxz + szx);
+    double dcFd1 = -two * dy * syy - dx * (sxy + syx) - dz * (syz + szy);
+    double dcFd2 = -two * dz * szz - dx * (sxz + szx) - dy * (syz + szy);
+    u[7] += (five * ccn1 * dx * dx - r2 * dx * dcFd0 - r2 * ccn1) * rinv7;
+    u[8] += (five * ccn1 * dx * dy - r2 * dx * dcFd1) * rinv7;
+    u[9] += (five * ccn1 * dx * dz - r2 * dx * dcFd2) * rinv7;
+    u[10] += (five * ccn1 * dy * dx - r2 * dy * dcFd0) * rinv7;
+    u[11] += (five * ccn1 * dy * dy - r2 * dy * dcFd1 - r2 * ccn1) * rinv7;
+    u[12] += (five * ccn1 * dy * dz - r2 * dy * dcFd2) * rinv7;
+    u[13] += (five * ccn1 * dz * dx - r2 * dz * dcFd0) * rinv7;
+    u[14] += (five * ccn1 * dz * dy - r2 * dz * dcFd1) * rinv7;
+    u[15] += (five * ccn1 * dz * dz - r2 * dz * dcFd2 - r2 * ccn1) * rinv7;
+}
+
+/* stokes_doublelaplacian::uKerEval, StokesDoubleLayerKernel.hpp:157-205; D (9) -> 7 */
+static inline void uk_stokes_doublelaplacian(double *u, const double *r, const double *f) {
+    const double dx = r[0], dy = r[1], dz = r[2];
+    double r2 = dx * dx + dy * dy + dz * dz;
+    double rinv = rsqrt_masked(r2);
+    double rinv2 = rinv * rinv;
+    double rinv3 = rinv * rinv2;
+    double rinv5 = rinv3 * rinv2;
+    double rinv7 = rinv5 * rinv2;
+    const double two = 2.0, three = 3.0;
+    struct dl_terms t = dl_common(r, f);
+    u[0] += two * (t.ccn3 + r2 * t.trace) * rinv5;
+    u[1] += rinv5 * dx * t.ccn3;
+    u[2] += rinv5 * dy * t.ccn3;
+    u[3] += rinv5 * dz * t.ccn3;
+    rinv7 *= -three;
+    u[4] -= two * rinv7 * (dx * t.cc5 - r2 * ((t.rksxk + t.rkskx) + dx * t.trace));
+    u[5] -= two * rinv7 * (dy * t.cc5 - r2 * ((t.rksyk + t.rksky) + dy * t.trace));
+    u[6] -= two * rinv7 * (dz * t.cc5 - r2 * ((t.rkszk + t.rkskz) + dz * t.trace));
+}
+
+/* stokes_doubletraction::uKerEval, StokesDoubleLayerKernel.hpp:220-272 (ScaleFactor -3/(8 pi), :217,
+ * the reference's constant facp = 0.66666666666666, 14 digits); D (9) -> 9 */
+static inline void uk_stokes_doubletraction(double *u, const double *r, const double *f) {
+    const double sxx = f[0], sxy = f[1], sxz = f[2];
+    const double syx = f[3], syy = f[4], syz = f[5];
+    const double szx = f[6], szy = f[7], szz = f[8];
+    const double dx = r[0], dy = r[1], dz = r[2];
+    double r2 = dx * dx + dy * dy + dz * dz;
+    double rinv = rsqrt_masked(r2);
+    double rinv2 = rinv * rinv;
+    double rinv3 = rinv * rinv2;
+    double rinv5 = rinv3 * rinv2;
+    double rinv7 = rinv5 * rinv2;
+    const double two = 2.0, five = 5.0, facp = 0.66666666666666;
+    struct dl_terms t = dl_common(r, f);
+    double dcFd0 = -two * dx * sxx - dy * (sxy + syx) - dz * (sxz + szx);
+    double dcFd1 = -two * dy * syy - dx * (sxy + syx) - dz * (syz + szy);
+    double dcFd2 = -two * dz * szz - dx * (sxz + szx) - dy * (syz + szy);
+    double np = r2 * facp * rinv7 * (t.ccn3 + r2 * t.trace);
+    double tv0 = (-five * t.cc * dx * dx - r2 * dx * dcFd0 + r2 * t.cc) * rinv7;
+    double tv1 = (-five * t.cc * dx * dy - r2 * dx * dcFd1) * rinv7;
+    double tv2 = (-five * t.cc * dx * dz - r2 * dx * dcFd2) * rinv7;
+    double tv3 = (-five * t.cc * dy * dx - r2 * dy * dcFd0) * rinv7;
+    double tv4 = (-five * t.cc * dy * dy - r2 * dy * dcFd1 + r2 * t.cc) * rinv7;
+    double tv5 = (-five * t.cc * dy * dz - r2 * dy * dcFd2) * rinv7;
+    double tv6 = (-five * t.cc * dz * dx - r2 * dz * dcFd0) * rinv7;
+    double tv7 = (-five * t.cc * dz * dy - r2 * dz * dcFd1) * rinv7;
+    double tv8 = (-five * t.cc * dz * dz - r2 * dz * dcFd2 + r2 * t.cc) * rinv7;
+    u[0] += np + tv0 + tv0;
+    u[1] += tv1 + tv3;
+    u[2] += tv2 + tv6;
+    u[3] += tv1 + tv3;
+    u[4] += np + tv4 + tv4;
+    u[5] += tv5 + tv7;
+    u[6] += tv2 + tv6;
+    u[7] += tv5 + tv7;
+    u[8] += np + tv8 + tv8;
+}
+
+/* ---- regularised force/torque kernels (StokesRegSingleLayerKernel.hpp) ------------------------
+ * The reference calls approx_rsqrt(eps^2 + r^2) without a mask (:70,121): with eps = 0 at a
+ * coincident point it divides by zero.  Here (and in the GPU kernels) that one case gives 0, like
+ * every other kernel's coincident pair; for eps > 0 or r > 0 nothing changes. */
+
+/* stokes_regftvel::uKerEval, StokesRegSingleLayerKernel.hpp:55-88; [f(3), t(3), eps] -> u(3) */
+static inline void uk_stokes_regftvel(double *u, const double *r, const double *f) {
+    const double half = 0.5, two = 2.0, five = 5.0;
+    const double dx = r[0], dy = r[1], dz = r[2];
+    const double fx = f[0], fy = f[1], fz = f[2];
+    const double tx = f[3], ty = f[4], tz = f[5];
+    const double eps = f[6];
+    double eps2 = eps * eps;
+    double r2 = dx * dx + dy * dy + dz * dz;
+    double denom_arg = r2 + eps2;
+    double rinv = rsqrt_masked(denom_arg);
+    double stokeslet_denom_inv = rinv * rinv * rinv;
+    double rotlet_denom_inv = half * stokeslet_denom_inv * rinv * rinv;
+    double rotlet_coef = (two * r2 + five * eps2) * rotlet_denom_inv;
+    double H2 = stokeslet_denom_inv;
+    double H1 = (r2 + two * eps2) * H2;
+    double tcurlrx = ty * dz - tz * dy;
+    double tcurlry = tz * dx - tx * dz;
+    double tcurlrz = tx * dy - ty * dx;
+    double fdotr = fx * dx + fy * dy + fz * dz;
+    u[0] += H1 * fx + H2 * fdotr * dx + rotlet_coef * tcurlrx;
+    u[1] += H1 * fy + H2 * fdotr * dy + rotlet_coef * tcurlry;
+    u[2] += H1 * fz + H2 * fdotr * dz + rotlet_coef * tcurlrz;
+}
+
+/* stokes_regftvelomega::uKerEval, StokesRegSingleLayerKernel.hpp:103-150;
+ * [f(3), t(3), eps] -> [u(3), omega(3)] */
+static inline void uk_stokes_regftvelomega(double *u, const double *r, const double *f) {
+    double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    const double half = 0.5, two = 2.0, three = 3.0, five = 5.0, seven = 7.0;
+    const double dx = r[0], dy = r[1], dz = r[2];
+    const double fx = f[0], fy = f[1], fz = f[2];
+    const double tx = f[3], ty = f[4], tz = f[5];
+    const double eps = f[6];
+    double r4 = r2 * r2;
+    double eps2 = eps * eps;
+    double eps4 = eps2 * eps2;
+    double rinv = rsqrt_masked(eps2 + r2);
+    const double stokeslet_denom_inv = rinv * rinv * rinv;
+    double rotlet_denom_inv = half * stokeslet_denom_inv * rinv * rinv;
+    double dipole_denom_inv = half * rotlet_denom_inv * rinv * rinv;
+    double rotlet_coef = (two * r2 + five * eps2) * rotlet_denom_inv;
+    double D1 = (two * five * eps4 - seven * eps2 * r2 - two * r4) * dipole_denom_inv;
+    double D2 = (seven * three * eps2 + two * three * r2) * dipole_denom_inv;
+    const double H2 = stokeslet_denom_inv;
+    double H1 = (r2 + two * eps2) * H2;
+    double fcurlrx = fy * dz - fz * dy;
+    double fcurlry = fz * dx - fx * dz;
+    double fcurlrz = fx * dy - fy * dx;
+    double tcurlrx = ty * dz - tz * dy;
+    double tcurlry = tz * dx - tx * dz;
+    double tcurlrz = tx * dy - ty * dx;
+    double fdotr = fx * dx + fy * dy + fz * dz;
+    double tdotr = tx * dx + ty * dy + tz * dz;
+    u[0] += H1 * fx + H2 * fdotr * dx + rotlet_coef * tcurlrx;
+    u[1] += H1 * fy + H2 * fdotr * dy + rotlet_coef * tcurlry;
+    u[2] += H1 * fz + H2 * fdotr * dz + rotlet_coef * tcurlrz;
+    u[3] += D1 * tx + D2 * tdotr * dx + rotlet_coef * fcurlrx;
+    u[4] += D1 * ty + D2 * tdotr * dy + rotlet_coef * fcurlry;
+    u[5] += D1 * tz + D2 * tdotr * dz + rotlet_coef * fcurlrz;
+}
+
+/* stokes_velomega::uKerEval, StokesRegSingleLayerKernel.hpp:165-201; f(3) -> [u(3), omega(3)]
+ * (the L2T kernel of StokesRegVelOmega, :223-229; its unused `reg = f[3]` read, :176, is dropped) */
+static inline void uk_stokes_velomega(double *u, const double *r, const double *f) {
+    double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    double rinv = rsqrt_masked(r2);
+    double rinv3 = rinv * rinv * rinv;
+    const double dx = r[0], dy = r[1], dz = r[2];
+    const double fx = f[0], fy = f[1], fz = f[2];
+    const double stokeslet_denom_inv = rinv3;
+    double rotlet_denom_inv = 0.5 * stokeslet_denom_inv * rinv * rinv;
+    double rotlet_coef = 2.0 * r2 * rotlet_denom_inv;
+    double H2 = stokeslet_denom_inv;
+    double H1 = r2 * H2;
+    double fcurlrx = fy * dz - fz * dy;
+    double fcurlry = fz * dx - fx * dz;
+    double fcurlrz = fx * dy - fy * dx;
+    double fdotr = fx * dx + fy * dy + fz * dz;
+    u[0] += H1 * fx + H2 * fdotr * dx;
+    u[1] += H1 * fy + H2 * fdotr * dy;
+    u[2] += H1 * fz + H2 * fdotr * dz;
+    u[3] += rotlet_coef * fcurlrx;
+    u[4] += rotlet_coef * fcurlry;
+    u[5] += rotlet_coef * fcurlrz;
+}
+
+/* ---- Laplace quadrupole (LaplaceLayerKernel.hpp) -------------------------------------------- */
+
+/* laplace_quadp::uKerEval, LaplaceLayerKernel.hpp:209-231; Q row-major (9) -> p */
+static inline void uk_laplace_quadp(double *u, const double *r, const double *f) {
+    double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    double rinv = rsqrt_masked(r2);
+    double rinv5 = rinv * rinv;
+    rinv5 = rinv5 * rinv5 * rinv;
+    const double sxx = f[0], sxy = f[1], sxz = f[2];
+    const double syx = f[3], syy = f[4], syz = f[5];
+    const double szx = f[6], szy = f[7], szz = f[8];
+    const double dx = r[0], dy = r[1], dz = r[2];
+    double cc = sxx * dx * dx;
+    cc += (sxy + syx) * dx * dy;
+    cc += (sxz + szx) * dx * dz;
+    cc += syy * dy * dy;
+    cc += (syz + szy) * dy * dz;
+    cc += szz * dz * dz;
+    cc *= 3.0;
+    u[0] += (cc - r2 * (sxx + syy + szz)) * rinv5;
+}
+
+/* laplace_quadpgradgrad::uKerEval, LaplaceLayerKernel.hpp:241-311; Q (9) -> [p, grad(3),
+ * Hessian xx,xy,xz,yy,yz,zz] */
+static inline void uk_laplace_quadpgradgrad(double *u, const double *r, const double *f) {
+    double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    double rinv = rsqrt_masked(r2);
+    double rinv2 = rinv * rinv;
+    double rinv3 = rinv2 * rinv;
+    double rinv5 = rinv3 * rinv2;
+    double rinv7 = rinv5 * rinv2;
+    double rinv9 = rinv7 * rinv2;
+    const double two = 2.0, three = 3.0, five = 5.0, fifteen = 15.0, onezerofive = 105.0;
+    const double sxx = f[0], sxy = f[1], sxz = f[2];
+    const double syx = f[3], syy = f[4], syz = f[5];
+    const double szx = f[6], szy = f[7], szz = f[8];
+    const double dx = r[0], dy = r[1], dz = r[2];
+    double rrQ = dx * dx * sxx + dx * dy * (sxy + syx) + dx * dz * (sxz + szx) + dy * dy * syy +
+                 dy * dz * (syz + szy) + dz * dz * szz;
+    double cc3 = rrQ * three;
+    double cc5 = rrQ * five;
+    double trace = sxx + syy + szz;
+    double rksxk = dx * sxx + dy * sxy + dz * sxz;
+    double rksyk = dx * syx + dy * syy + dz * syz;
+    double rkszk = dx * szx + dy * szy + dz * szz;
+    double rkskx = dx * sxx + dy * syx + dz * szx;
+    double rksky = dx * sxy + dy * syy + dz * szy;
+    double rkskz = dx * sxz + dy * syz + dz * szz;
+    u[0] += (cc3 - r2 * trace) * rinv5;
+    double px = dx * cc5 - r2 * (rksxk + rkskx + dx * trace);
+    double py = dy * cc5 - r2 * (rksyk + rksky + dy * trace);
+    double pz = dz * cc5 - r2 * (rkszk + rkskz + dz * trace);
+    u[1] -= three * px * rinv7;
+    u[2] -= three * py * rinv7;
+    u[3] -= three * pz * rinv7;
+    u[4] += rinv9 * (onezerofive * dx * dx * rrQ - r2 * fifteen * (rrQ + dx * two * (rksxk + rkskx) + dx * dx * trace) +
+                     three * r2 * r2 * (trace + two * sxx));
+    u[5] += rinv9 * (onezerofive * (dx * dy * rrQ) -
+                     r2 * fifteen * ((dy * rksxk + dy * rkskx + dx * rksyk + dx * rksky) + dx * dy * trace) +
+                     three * r2 * r2 * (sxy + syx));
+    u[6] += rinv9 * (onezerofive * (dx * dz * rrQ) -
+                     r2 * fifteen * ((dz * rksxk + dz * rkskx + dx * rkszk + dx * rkskz) + dx * dz * trace) +
+                     three * r2 * r2 * (sxz + szx));
+    u[7] += rinv9 * (onezerofive * dy * dy * rrQ - r2 * fifteen * (rrQ + dy * two * (rksyk + rksky) + dy * dy * trace) +
+                     three * r2 * r2 * (trace + two * syy));
+    u[8] += rinv9 * (onezerofive * (dy * dz * rrQ) -
+                     r2 * fifteen * ((dy * rkszk + dy * rkskz + dz * rksyk + dz * rksky) + dy * dz * trace) +
+                     three * r2 * r2 * (syz + szy));
+    u[9] += rinv9 * (onezerofive * dz * dz * rrQ - r2 * fifteen * (rrQ + dz * two * (rkszk + rkskz) + dz * dz * trace) +
+                     three * r2 * r2 * (trace + two * szz));
+}
+
 typedef void (*uk_fn)(double *, const double *, const double *);
 static const uk_fn uk_table[UK_COUNT] = {
     uk_laplace_p,         uk_laplace_dipolep,   uk_laplace_pgrad, uk_laplace_dipolepgrad,
     uk_laplace_pgradgrad, uk_laplace_dipolepgradgrad, uk_stokes_vel, uk_stokes_pvel,
     uk_stokes_doublepvel, uk_stokes_regvel,     uk_rpy_u,         uk_rpy_ulapu,
-    uk_stk_ulapu,         uk_stokes_pvel_fused};
+    uk_stk_ulapu,         uk_stokes_pvel_fused, uk_stokes_pvelgrad, uk_stokes_doublepvelgrad,
+    uk_stokes_pvellaplacian, uk_stokes_doublelaplacian, uk_stokes_traction, uk_stokes_doubletraction,
+    uk_stokes_regftvelomega, uk_stokes_velomega, uk_laplace_quadpgradgrad, uk_laplace_quadp,
+    uk_stokes_regftvel};
 
 /* one pair, no scale factor; u is overwritten (zeroed first) */
 int oracle_uker_pair(int uk, const double *r, const double *f, double *u) {

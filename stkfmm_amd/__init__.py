@@ -70,6 +70,17 @@ class UKER(enum.IntEnum):
     RPY_ULAPU = 11
     STK_ULAPU = 12
     STOKES_PVEL_FUSED = 13
+    STOKES_PVELGRAD = 14
+    STOKES_DOUBLEPVELGRAD = 15
+    STOKES_PVELLAPLACIAN = 16
+    STOKES_DOUBLELAPLACIAN = 17
+    STOKES_TRACTION = 18
+    STOKES_DOUBLETRACTION = 19
+    STOKES_REGFTVELOMEGA = 20
+    STOKES_VELOMEGA = 21
+    LAPLACE_QUADPGRADGRAD = 22
+    LAPLACE_QUADP = 23
+    STOKES_REGFTVEL = 24
 
 
 # ---- errors ---------------------------------------------------------------------------------------

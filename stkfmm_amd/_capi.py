@@ -32,7 +32,11 @@ class StkTune(ctypes.Structure):
 
 KER_T_NAMES = ["laplace_p", "laplace_dipolep", "laplace_pgrad", "laplace_dipolepgrad", "laplace_pgradgrad",
                "laplace_dipolepgradgrad", "stokes_vel", "stokes_pvel", "stokes_doublepvel", "stokes_regvel",
-               "rpy_u", "rpy_ulapu", "stk_ulapu"]
+               "rpy_u", "rpy_ulapu", "stk_ulapu", "stokes_pvelgrad", "stokes_doublepvelgrad",
+               "stokes_pvellaplacian", "stokes_doublelaplacian", "stokes_traction", "stokes_doubletraction",
+               "stokes_regftvelomega", "stokes_velomega", "laplace_quadpgradgrad", "laplace_quadp", "stokes_regftvel"]
+# micro-kernel id (STKFMM_UKER_*) of each Ker_t entry; 13 is the fused PVel kernel, which has no Ker_t entry
+KER_T_UKER = {n: (i if i < 13 else i + 1) for i, n in enumerate(KER_T_NAMES)}
 
 # every symbol include/stkfmm_p2p.h declares
 EXPORTS = [

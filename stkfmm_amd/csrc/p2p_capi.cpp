@@ -67,15 +67,18 @@ const KernelEntry kTable[] = {
     {1, "laplace_PGrad", 1, 3, 4, STKFMM_UKER_LAPLACE_PGRAD, STKFMM_UKER_LAPLACE_DIPOLEPGRAD, STKFMM_UKER_LAPLACE_PGRAD},
     {2, "laplace_PGradGrad", 1, 3, 10, STKFMM_UKER_LAPLACE_PGRADGRAD, STKFMM_UKER_LAPLACE_DIPOLEPGRADGRAD,
      STKFMM_UKER_LAPLACE_PGRADGRAD},
-    {4, "laplace_QPGradGrad", 9, 12, 10, -2, -1, STKFMM_UKER_LAPLACE_PGRADGRAD},
+    {4, "laplace_QPGradGrad", 9, 12, 10, STKFMM_UKER_LAPLACE_QUADPGRADGRAD, -1, STKFMM_UKER_LAPLACE_PGRADGRAD},
     {8, "stokes_vel", 3, 6, 3, STKFMM_UKER_STOKES_VEL, -1, STKFMM_UKER_STOKES_VEL},
     {16, "rpy_ulapu", 4, 7, 6, STKFMM_UKER_RPY_ULAPU, -1, STKFMM_UKER_STK_ULAPU},
     {32, "stokes_regvel", 4, 7, 3, STKFMM_UKER_STOKES_REGVEL, -1, STKFMM_UKER_STOKES_VEL},
-    {64, "stokes_regftvelomega", 7, 10, 6, -2, -1, -2},
+    {64, "stokes_regftvelomega", 7, 10, 6, STKFMM_UKER_STOKES_REGFTVELOMEGA, -1, STKFMM_UKER_STOKES_VELOMEGA},
     {128, "stokes_PVel", 4, 9, 4, STKFMM_UKER_STOKES_PVEL, STKFMM_UKER_STOKES_DOUBLEPVEL, STKFMM_UKER_STOKES_PVEL},
-    {256, "stokes_PVelGrad", 4, 9, 16, -2, -2, -2},
-    {512, "stokes_PVelLaplacian", 4, 9, 7, -2, -2, -2},
-    {1024, "stokes_Traction", 4, 9, 9, -2, -2, -2},
+    {256, "stokes_PVelGrad", 4, 9, 16, STKFMM_UKER_STOKES_PVELGRAD, STKFMM_UKER_STOKES_DOUBLEPVELGRAD,
+     STKFMM_UKER_STOKES_PVELGRAD},
+    {512, "stokes_PVelLaplacian", 4, 9, 7, STKFMM_UKER_STOKES_PVELLAPLACIAN, STKFMM_UKER_STOKES_DOUBLELAPLACIAN,
+     STKFMM_UKER_STOKES_PVELLAPLACIAN},
+    {1024, "stokes_Traction", 4, 9, 9, STKFMM_UKER_STOKES_TRACTION, STKFMM_UKER_STOKES_DOUBLETRACTION,
+     STKFMM_UKER_STOKES_TRACTION},
 };
 
 const KernelEntry *find_kernel(unsigned kernel) {
@@ -777,6 +780,17 @@ STKFMM_KER_T(stokes_regvel, STKFMM_UKER_STOKES_REGVEL)
 STKFMM_KER_T(rpy_u, STKFMM_UKER_RPY_U)
 STKFMM_KER_T(rpy_ulapu, STKFMM_UKER_RPY_ULAPU)
 STKFMM_KER_T(stk_ulapu, STKFMM_UKER_STK_ULAPU)
+STKFMM_KER_T(stokes_pvelgrad, STKFMM_UKER_STOKES_PVELGRAD)
+STKFMM_KER_T(stokes_doublepvelgrad, STKFMM_UKER_STOKES_DOUBLEPVELGRAD)
+STKFMM_KER_T(stokes_pvellaplacian, STKFMM_UKER_STOKES_PVELLAPLACIAN)
+STKFMM_KER_T(stokes_doublelaplacian, STKFMM_UKER_STOKES_DOUBLELAPLACIAN)
+STKFMM_KER_T(stokes_traction, STKFMM_UKER_STOKES_TRACTION)
+STKFMM_KER_T(stokes_doubletraction, STKFMM_UKER_STOKES_DOUBLETRACTION)
+STKFMM_KER_T(stokes_regftvelomega, STKFMM_UKER_STOKES_REGFTVELOMEGA)
+STKFMM_KER_T(stokes_velomega, STKFMM_UKER_STOKES_VELOMEGA)
+STKFMM_KER_T(laplace_quadpgradgrad, STKFMM_UKER_LAPLACE_QUADPGRADGRAD)
+STKFMM_KER_T(laplace_quadp, STKFMM_UKER_LAPLACE_QUADP)
+STKFMM_KER_T(stokes_regftvel, STKFMM_UKER_STOKES_REGFTVEL)
 
 // ---- adjoint ------------------------------------------------------------------------------------------
 

@@ -95,6 +95,26 @@ struct VjpArgs {
     double *g_tc;      // [n_trg*3]  (may be null)
 };
 
+// Every launch entry of one micro-kernel (all PREC x T instantiations), filled by
+// make_uker_ops<UK>() in the translation unit that instantiates that micro-kernel's kernels.
+struct UkerOps {
+    hipError_t (*launch)(int prec, int T, int grid, const ItemArgs &a, hipStream_t s);
+    hipError_t (*occ)(int prec, int T, int *blocks);
+    hipError_t (*launch_sgpr)(int prec, int T, int grid, const ItemArgs &a, const double *recs, int level_min,
+                              hipStream_t s);
+    hipError_t (*occ_sgpr)(int prec, int T, int *blocks);
+    hipError_t (*stage)(int prec, const StageArgs &a, int n_copies, hipStream_t s);
+    int words[2]; // scalar-broadcast record size in 8-byte words, FP64 / mixed
+    int ks, ks1, kt;
+};
+
+constexpr int kNumCoreUker = 14;   // micro-kernels 0..13 (p2p_kernels.hip)
+constexpr int kNumUkerTotal = 25;  // 14..24 in p2p_family.hip
+
+// table row of micro-kernel uk, or nullptr
+const UkerOps *uker_ops(int uk);
+const UkerOps *uker_ops_family(int uk);
+
 // kernel-table queries (host)
 int uker_ks(int uk);
 int uker_ks1(int uk);

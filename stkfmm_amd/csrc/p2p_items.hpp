@@ -1,0 +1,300 @@
+// p2p_items.hpp -- the item kernel template and its per-micro-kernel launch helpers.
+//
+//   p2p_items_kernel<UK, PREC, T>  the hot kernel.  One wavefront per workgroup; each lane holds T
+//       targets in registers; the wave walks the source ranges of its work item, staging 64 sources
+//       at a time into LDS with coalesced loads (one source per lane), then every lane reads each
+//       staged source by broadcast and evaluates the micro-kernel against its T targets.  Sums go
+//       to per-item partial slots ([slot][KT][64*T], lane-contiguous, coalesced).
+//       Persistent mode: a grid sized to the device's occupancy; each wave takes the next item of
+//       the cost-sorted queue with one atomic on a counter, until the queue is empty (every wave
+//       reaches that exit).  Otherwise one workgroup per item, in queue order.
+//
+// PREC 0 = FP64 throughout.  PREC 1 = mixed: coordinates in FP64 scaled by 2^level, displacement
+// in FP64, micro-kernel (rsqrt + 1 Newton step, polynomial) in FP32, FP32 partial sums over
+// kMixedAccBlock sources added into FP64 per-target accumulators.
+//
+// make_uker_ops<UK>() bundles every launch entry of one micro-kernel (LDS and scalar-broadcast
+// item kernels for PREC x T, their occupancy queries, record staging, dimensions) into a UkerOps
+// table row; each translation unit instantiates the rows of its micro-kernels, so the kernel
+// family compiles in parallel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "p2p_internal.hpp"
+#include "p2p_sgpr.hpp"
+#include "ukernels.hpp"
+
+namespace stkp2p {
+
+// STKP2P_MIN_WAVES_PER_EU (build option): ask the compiler to keep at least this many waves of the
+// item kernel resident per SIMD (register budget 512 / waves).  0 leaves the choice to it.
+#ifndef STKP2P_MIN_WAVES_PER_EU
+#define STKP2P_MIN_WAVES_PER_EU 0
+#endif
+#if STKP2P_MIN_WAVES_PER_EU > 0
+#define STKP2P_ITEM_ATTR __attribute__((amdgpu_waves_per_eu(STKP2P_MIN_WAVES_PER_EU, 8)))
+#else
+#define STKP2P_ITEM_ATTR
+#endif
+
+template <int UK, int PREC, int T>
+__global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemArgs a) {
+    using K = typename UkerOf<UK>::type;
+    constexpr int KL = K::KL, KT = K::KT, KS = K::KS, KS1 = K::KS1;
+    constexpr int TT = kWave * T;
+
+    // FP64: values as [KL][tile] (two consecutive sources per 16-byte read);
+    // mixed: FP32 values as 16-byte aligned rows [tile][KLP] (one source per few 16-byte reads,
+    // all at immediate offsets from one base address)
+    constexpr int KLP = ((KL + 3) / 4) * 4;
+    __shared__ double s_x[kSrcTile], s_y[kSrcTile], s_z[kSrcTile];
+    __shared__ double s_vd[PREC == 0 ? KL : 1][kSrcTile];
+    __shared__ __attribute__((aligned(16))) float s_vf[PREC == 0 ? 1 : kSrcTile][KLP];
+
+    const int lane = threadIdx.x;
+    bool first = true;
+    for (;;) {
+        int it;
+        if (a.counter) {
+            int v = 0;
+            if (lane == 0)
+                v = atomicAdd(a.counter, 1);
+            it = __builtin_amdgcn_readfirstlane(v);
+        } else {
+            if (!first)
+                break;
+            it = blockIdx.x;
+        }
+        first = false;
+        if (it >= a.n_items)
+            break;
+
+        const Item item = a.items[it];
+        const double lam = (PREC == 0) ? 1.0 : ldexp(1.0, item.level);
+        // lanes per target (uniform): lane = target slot * split + sub
+        const int ls = item.lsplit;
+        const int split = 1 << ls;
+        const int sub = lane & (split - 1);
+        const int slot0 = lane >> ls;
+        const int tstride = kWave >> ls;
+
+        double tx[T], ty[T], tz[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const int i = slot0 + tstride * t;
+            const long long idx = item.trg_begin + ((i < item.trg_count) ? i : item.trg_count - 1);
+            tx[t] = a.tc.p[idx * a.tc.s] * lam;
+            ty[t] = a.tc.p[idx * a.tc.s + a.tc.fs] * lam;
+            tz[t] = a.tc.p[idx * a.tc.s + 2 * a.tc.fs] * lam;
+        }
+        double acc[T][KT];
+#pragma unroll
+        for (int t = 0; t < T; ++t)
+#pragma unroll
+            for (int k = 0; k < KT; ++k)
+                acc[t][k] = 0.0;
+
+        for (int r = item.rng_begin; r < item.rng_end; ++r) {
+            const int2 rg = a.ranges[r];
+            for (int base = rg.x; base < rg.y; base += kSrcTile) {
+                const int n = (rg.y - base < kSrcTile) ? rg.y - base : kSrcTile;
+                __syncthreads(); // the previous tile has been read by every lane
+                if (lane < n) {
+                    const long long s = base + lane;
+                    double x = a.sc.p[s * a.sc.s];
+                    double y = a.sc.p[s * a.sc.s + a.sc.fs];
+                    double z = a.sc.p[s * a.sc.s + 2 * a.sc.fs];
+                    double vin[KS];
+#pragma unroll
+                    for (int k = 0; k < KS; ++k)
+                        vin[k] = (k < KS1) ? a.v1.p[s * a.v1.s + k * a.v1.fs]
+                                           : a.v2.p[s * a.v2.s + (k - KS1) * a.v2.fs];
+                    if (PREC != 0) {
+                        x *= lam;
+                        y *= lam;
+                        z *= lam;
+#pragma unroll
+                        for (int k = 0; k < KS; ++k)
+                            if ((K::SRC_SCALE_MASK >> k) & 1u)
+                                vin[k] *= lam;
+                    }
+                    double vst[KL];
+                    K::stage(vin, vst);
+                    s_x[lane] = x;
+                    s_y[lane] = y;
+                    s_z[lane] = z;
+#pragma unroll
+                    for (int k = 0; k < KL; ++k) {
+                        if (PREC == 0)
+                            s_vd[k][lane] = vst[k];
+                        else
+                            s_vf[lane][k] = (float)vst[k];
+                    }
+                }
+                __syncthreads();
+                // sources of this tile: every lane takes all of them (split == 1, uniform loop with
+                // immediate LDS offsets) or every split-th one starting at its sub-lane
+                auto tile_pairs = [&](auto strided) {
+                    constexpr bool STR = decltype(strided)::value;
+                    const int j_first = STR ? sub : 0;
+                    const int j_step = STR ? split : 1;
+                    if (PREC == 0) {
+#pragma unroll 2
+                        for (int j = j_first; j < n; j += j_step) {
+                            const double sx = s_x[j], sy = s_y[j], sz = s_z[j];
+                            double f[KL];
+#pragma unroll
+                            for (int k = 0; k < KL; ++k)
+                                f[k] = s_vd[k][j];
+#pragma unroll
+                            for (int t = 0; t < T; ++t)
+                                K::template eval<double>(acc[t], tx[t] - sx, ty[t] - sy, tz[t] - sz, f);
+                        }
+                    } else {
+                        for (int j0 = 0; j0 < n; j0 += kMixedAccBlock) {
+                            const int j1 = (n - j0 < kMixedAccBlock) ? n : j0 + kMixedAccBlock;
+                            float part[T][KT];
+#pragma unroll
+                            for (int t = 0; t < T; ++t)
+#pragma unroll
+                                for (int k = 0; k < KT; ++k)
+                                    part[t][k] = 0.0f;
+#pragma unroll 2
+                            for (int j = j0 + j_first; j < j1; j += j_step) {
+                                const double sx = s_x[j], sy = s_y[j], sz = s_z[j];
+                                float f[KL];
+#pragma unroll
+                                for (int k = 0; k < KL; ++k)
+                                    f[k] = s_vf[j][k];
+#pragma unroll
+                                for (int t = 0; t < T; ++t)
+                                    K::template eval<float>(part[t], (float)(tx[t] - sx), (float)(ty[t] - sy),
+                                                            (float)(tz[t] - sz), f);
+                            }
+#pragma unroll
+                            for (int t = 0; t < T; ++t)
+#pragma unroll
+                                for (int k = 0; k < KT; ++k)
+                                    acc[t][k] += (double)part[t][k];
+                        }
+                    }
+                };
+                if (ls == 0)
+                    tile_pairs(std::false_type{});
+                else
+                    tile_pairs(std::true_type{});
+            }
+        }
+
+        // lanes that share a target add their partial sums (butterfly over the low ls lane bits)
+        for (int off = 1; off < split; off <<= 1) {
+#pragma unroll
+            for (int t = 0; t < T; ++t)
+#pragma unroll
+                for (int k = 0; k < KT; ++k)
+                    acc[t][k] += __shfl_xor(acc[t][k], off);
+        }
+        double *dst = a.partial + (long long)item.slot * KT * TT;
+#pragma unroll
+        for (int k = 0; k < KT; ++k) {
+            const double osc = ((PREC == 0) ? 1.0 : ldexp(1.0, K::deg(k) * item.level)) * K::post(k);
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const int i = slot0 + tstride * t;
+                if (sub == 0 && i < item.trg_count)
+                    dst[(long long)k * TT + i] = acc[t][k] * osc;
+            }
+        }
+    }
+}
+
+// ---- per-micro-kernel launch helpers -----------------------------------------------------------
+
+template <int UK, int PREC, int T>
+hipError_t launch_items_one(int grid, const ItemArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL((p2p_items_kernel<UK, PREC, T>), dim3(grid), dim3(kWave), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int UK, int PREC, int T>
+hipError_t occ_items_one(int *blocks) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, p2p_items_kernel<UK, PREC, T>, kWave, 0);
+}
+
+template <int UK, int PREC, int T>
+hipError_t launch_sgpr_one(int grid, const ItemArgs &a, const double *recs, int level_min, hipStream_t s) {
+    hipLaunchKernelGGL((p2p_items_sgpr_kernel<UK, PREC, T>), dim3(grid), dim3(kWave), 0, s, a, recs, level_min);
+    return hipGetLastError();
+}
+
+template <int UK, int PREC, int T>
+hipError_t occ_sgpr_one(int *blocks) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, p2p_items_sgpr_kernel<UK, PREC, T>, kWave, 0);
+}
+
+// PREC in {0, 1} x T in {1, 2, 4}
+#define STKP2P_PT_SWITCH(FN, ...)                                                                  \
+    switch (prec * 8 + T) {                                                                        \
+    case 1: return FN<UK, 0, 1>(__VA_ARGS__);                                                      \
+    case 2: return FN<UK, 0, 2>(__VA_ARGS__);                                                      \
+    case 4: return FN<UK, 0, 4>(__VA_ARGS__);                                                      \
+    case 9: return FN<UK, 1, 1>(__VA_ARGS__);                                                      \
+    case 10: return FN<UK, 1, 2>(__VA_ARGS__);                                                     \
+    case 12: return FN<UK, 1, 4>(__VA_ARGS__);                                                     \
+    default: return hipErrorInvalidValue;                                                          \
+    }
+
+template <int UK>
+hipError_t launch_items_uk(int prec, int T, int grid, const ItemArgs &a, hipStream_t s) {
+    STKP2P_PT_SWITCH(launch_items_one, grid, a, s)
+}
+
+template <int UK>
+hipError_t occ_items_uk(int prec, int T, int *blocks) {
+    STKP2P_PT_SWITCH(occ_items_one, blocks)
+}
+
+template <int UK>
+hipError_t launch_sgpr_uk(int prec, int T, int grid, const ItemArgs &a, const double *recs, int lmin,
+                          hipStream_t s) {
+    STKP2P_PT_SWITCH(launch_sgpr_one, grid, a, recs, lmin, s)
+}
+
+template <int UK>
+hipError_t occ_sgpr_uk(int prec, int T, int *blocks) {
+    STKP2P_PT_SWITCH(occ_sgpr_one, blocks)
+}
+
+#undef STKP2P_PT_SWITCH
+
+template <int UK>
+hipError_t stage_records_uk(int prec, const StageArgs &a, int n_copies, hipStream_t s) {
+    const dim3 grid((unsigned)((a.n_src + 255) / 256), (unsigned)n_copies);
+    if (prec == 0)
+        hipLaunchKernelGGL((stage_records_kernel<UK, 0>), grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((stage_records_kernel<UK, 1>), grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int UK>
+UkerOps make_uker_ops() {
+    using K = typename UkerOf<UK>::type;
+    UkerOps o;
+    o.launch = launch_items_uk<UK>;
+    o.occ = occ_items_uk<UK>;
+    o.launch_sgpr = launch_sgpr_uk<UK>;
+    o.occ_sgpr = occ_sgpr_uk<UK>;
+    o.stage = stage_records_uk<UK>;
+    o.words[0] = RecLayout<K, 0>::WORDS;
+    o.words[1] = RecLayout<K, 1>::WORDS;
+    o.ks = K::KS;
+    o.ks1 = K::KS1;
+    o.kt = K::KT;
+    return o;
+}
+
+} // namespace stkp2p

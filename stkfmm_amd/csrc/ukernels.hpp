@@ -484,6 +484,6 @@ template <> struct UkerOf<11> { using type = UkRpyULapU; };
 template <> struct UkerOf<12> { using type = UkStkULapU; };
 template <> struct UkerOf<13> { using type = UkStokesPVelFused; };
 
-constexpr int kNumUker = 14;
+constexpr int kNumUker = 14; // micro-kernels defined in this header (0..13)
 
 } // namespace stkp2p

@@ -28,6 +28,12 @@ CASES = [
     (16, 0, 4, 6, "radius"),
     (32, 0, 4, 3, "radius"),
     (128, 0, 4, 4, "uniform"), (128, 1, 9, 4, "uniform"),
+    # appended after the first set (the random stream of the cases above is unchanged)
+    (256, 0, 4, 16, "uniform"), (256, 1, 9, 16, "uniform"),
+    (512, 0, 4, 7, "uniform"), (512, 1, 9, 7, "uniform"),
+    (1024, 0, 4, 9, "uniform"), (1024, 1, 9, 9, "uniform"),
+    (64, 0, 7, 6, "radius"),
+    (4, 0, 9, 10, "uniform"),
 ]
 
 FIXED = np.array([[0.3, 0.2, 0.1], [0.2, 0.4, 0.3],                        # fixedPoints(2)

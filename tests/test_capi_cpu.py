@@ -41,9 +41,14 @@ def test_kernel_dimensions_and_names(k):
     assert S.get_kernel_name(k) == NAMES[k]
 
 
-# FMMData::evaluateKernel sub-kernel selection (Lib/src/FMMData.cpp:261-268) with the BuildKernel lists
+# FMMData::evaluateKernel sub-kernel selection (Lib/src/FMMData.cpp:261-268) with the BuildKernel lists:
+# k_l2t of LapQPGradGrad is laplace_pgradgrad (LaplaceLayerKernel.hpp:349-352), of StokesRegVelOmega
+# stokes_velomega (StokesRegSingleLayerKernel.hpp:227-229); PVelGrad / PVelLaplacian / Traction use
+# themselves (NULL in BuildKernel, StokesLayerKernel.hpp:53-89).
 UKERS = {(1, 1): 2, (1, 2): 3, (1, 4): 2, (2, 1): 4, (2, 2): 5, (2, 4): 4, (8, 1): 6, (8, 4): 6, (16, 1): 11,
-         (16, 4): 12, (32, 1): 9, (32, 4): 6, (128, 1): 7, (128, 2): 8, (128, 4): 7}
+         (16, 4): 12, (32, 1): 9, (32, 4): 6, (128, 1): 7, (128, 2): 8, (128, 4): 7,
+         (4, 1): 22, (4, 4): 4, (64, 1): 20, (64, 4): 21, (256, 1): 14, (256, 2): 15, (256, 4): 14,
+         (512, 1): 16, (512, 2): 17, (512, 4): 16, (1024, 1): 18, (1024, 2): 19, (1024, 4): 18}
 
 
 @pytest.mark.parametrize("kp", sorted(UKERS))
@@ -51,7 +56,7 @@ def test_sub_kernel_selection(kp):
     assert int(S.get_uker(*kp)) == UKERS[kp]
 
 
-@pytest.mark.parametrize("kp", [(8, 2), (16, 2), (32, 2)])
+@pytest.mark.parametrize("kp", [(8, 2), (16, 2), (32, 2), (4, 2), (64, 2)])
 def test_missing_double_layer(kp):
     with pytest.raises(S.PPKernelNotFound, match="not found for direct evaluation"):
         S.get_uker(*kp)
@@ -66,18 +71,22 @@ def test_unknown_kernel_and_ppkernel():
         S.get_uker(1, 3)
 
 
-@pytest.mark.parametrize("k", [4, 64, 256, 512, 1024])
-def test_unimplemented_kernels_raise(k):
-    with pytest.raises(S.Unsupported):
+def test_every_kernel_of_the_reference_map_is_built():
+    # every KERNEL of kernelMap (Lib/src/STKFMM.cpp:18-31) has its single-layer and L2T sub-kernels
+    for k in DIMS:
         S.get_uker(k, 1)
+        S.get_uker(k, 4)
 
 
 def test_uker_dims():
+    from oracle import oracle as O
     expect = [(1, 1), (3, 1), (1, 4), (3, 4), (1, 10), (3, 10), (3, 3), (4, 4), (9, 4), (4, 3), (4, 3), (4, 6),
-              (3, 6), (13, 4)]
-    assert [S.uker_dims(u) for u in range(14)] == expect
+              (3, 6), (13, 4), (4, 16), (9, 16), (4, 7), (9, 7), (4, 9), (9, 9), (7, 6), (3, 6), (9, 10), (9, 1),
+              (7, 3)]
+    assert [S.uker_dims(u) for u in range(25)] == expect
+    assert [(O.src_dim(u), O.trg_dim(u)) for u in range(25)] == expect
     with pytest.raises(S.ArgumentError):
-        S.uker_dims(14)
+        S.uker_dims(25)
 
 
 def test_empty_inputs_are_a_no_op():

@@ -13,7 +13,9 @@ from tests.gpu_util import FP64_TOL, MIXED_TOL, host, say, to_dev, zeros
 pytestmark = pytest.mark.gpu
 
 UKERS = {(1, 1): 2, (1, 2): 3, (1, 4): 2, (2, 1): 4, (2, 2): 5, (2, 4): 4, (8, 1): 6, (8, 4): 6, (16, 1): 11,
-         (16, 4): 12, (32, 1): 9, (32, 4): 6, (128, 1): 7, (128, 2): 8, (128, 4): 7}
+         (16, 4): 12, (32, 1): 9, (32, 4): 6, (128, 1): 7, (128, 2): 8, (128, 4): 7,
+         (4, 1): 22, (4, 4): 4, (64, 1): 20, (64, 4): 21, (256, 1): 14, (256, 2): 15, (256, 4): 14,
+         (512, 1): 16, (512, 2): 17, (512, 4): 16, (1024, 1): 18, (1024, 2): 19, (1024, 4): 18}
 
 
 def case(uk, seed, ns=700, nt=333, n_coinc=16):
@@ -24,7 +26,7 @@ def case(uk, seed, ns=700, nt=333, n_coinc=16):
 
 
 @pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
-@pytest.mark.parametrize("uk", range(14))
+@pytest.mark.parametrize("uk", range(25))
 def test_uker_direct_vs_oracle(uk, prec):
     say(f"uker={uk} prec={prec} ns=700 nt=333")
     src, v, trg = case(uk, 1000 + uk)
@@ -66,7 +68,7 @@ def test_golden_fixtures_through_gpu(golden, uk):
 
 @pytest.mark.parametrize("name", S._capi.KER_T_NAMES)
 def test_ker_t_entries(name):
-    uk = S._capi.KER_T_NAMES.index(name)
+    uk = S._capi.KER_T_UKER[name]
     say(f"Ker_t stkfmm_ker_{name}")
     src, v, trg = case(uk, 3000 + uk, ns=300, nt=200)
     v = np.ascontiguousarray(v)

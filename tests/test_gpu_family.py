@@ -1,0 +1,81 @@
+"""GPU parity for the rest of the kernel family (PVelGrad, PVelLaplacian, Traction single and double
+layer, StokesRegVelOmega and its sub-kernels, LapQPGradGrad and laplace_quadp) over octree U-lists,
+against the oracle's range-list driver.
+
+The point sets are clustered so that the tree is deep (level scaling in the mixed mode) and hold
+pairs far closer than the leaf size, where the reference's r^-7 / r^-9 factors would overflow FP32
+if formed literally."""
+import numpy as np
+import pytest
+
+import stkfmm_amd as S
+from oracle import oracle as O
+from stkfmm_amd import workloads as W
+from tests.conftest import RADIUS_UKERS
+from tests.gpu_util import FP64_TOL, MIXED_TOL, host, say, to_dev, zeros
+
+pytestmark = pytest.mark.gpu
+
+FAMILY = list(range(14, 25))
+
+
+def family_values(uk, n, rng):
+    v = rng.normal(size=(n, O.src_dim(uk)))
+    if uk in RADIUS_UKERS:
+        v[:, -1] = rng.uniform(1e-4, 5e-4, n)
+    return v
+
+
+def clustered_points(seed, n_uniform=4000, n_cluster=4000, sigma=0.01, n_tight=64, tight=1e-6):
+    g = np.random.default_rng(seed)
+    parts = [g.random((n_uniform, 3)), 0.4 + sigma * g.normal(size=(n_cluster, 3))]
+    # a few near-coincident pairs: offsets of ~1e-6 from existing points
+    base = g.random((n_tight, 3))
+    parts += [base, base + tight * g.normal(size=(n_tight, 3))]
+    return W.wrap(np.concatenate(parts))
+
+
+def run(uk, prec, bc, x, cap, seed):
+    tr = S.Tree(x, cap)
+    xs = x[tr.perm]
+    v = family_values(uk, tr.n, np.random.default_rng(seed))
+    plan = S.Plan.from_tree(tr, uk, prec, tune=dict(broadcast=bc))
+    out = zeros(tr.n, O.trg_dim(uk))
+    d_x = to_dev(xs)
+    plan.execute(d_x, to_dev(v), d_x, out)
+    ref = O.ranges(uk, xs, v, xs, tr.leaf_begin, tr.leaf_end, tr.ulist_ptr, tr.rng_begin, tr.rng_end)
+    return host(out), ref, tr
+
+
+@pytest.mark.parametrize("bc", [0, 1], ids=["lds", "scalar"])
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("uk", FAMILY)
+def test_family_ulist_vs_oracle(uk, prec, bc):
+    say(f"family U-list uker={uk} prec={prec} broadcast={bc}")
+    x = clustered_points(700 + uk)
+    out, ref, tr = run(uk, prec, bc, x, 120, 800 + uk)
+    assert tr.max_level >= 5
+    assert np.all(np.isfinite(out))
+    l2, mx = O.max_error(out, ref)
+    tol = FP64_TOL if prec == 0 else MIXED_TOL
+    assert l2 < tol and mx < tol, (l2, mx)
+
+
+@pytest.mark.parametrize("uk", FAMILY)
+def test_family_tuning_knobs_agree(uk):
+    say(f"family tuning uker={uk}")
+    x = clustered_points(900 + uk, n_uniform=2000, n_cluster=2000)
+    tr = S.Tree(x, 100)
+    xs = x[tr.perm]
+    v = family_values(uk, tr.n, np.random.default_rng(uk))
+    d_x, d_v = to_dev(xs), to_dev(v)
+    outs = []
+    for tune in (dict(targets_per_lane=1), dict(targets_per_lane=2, persistent=2),
+                 dict(targets_per_lane=4, max_src_per_item=512), dict(split_tiles=-1)):
+        plan = S.Plan.from_tree(tr, uk, S.PREC.FP64, tune=tune)
+        out = zeros(tr.n, O.trg_dim(uk))
+        plan.execute(d_x, d_v, d_x, out)
+        outs.append(host(out))
+    for o in outs[1:]:
+        l2, mx = O.max_error(o, outs[0])
+        assert l2 < 1e-13 and mx < 1e-13
