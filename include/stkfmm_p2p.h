@@ -275,7 +275,27 @@ int stkfmm_tree_info(const stkfmm_tree *tree, int *n_leaves, int *n_ranges, int 
  * rng_begin/rng_end [n_ranges] (sorted-order indices) */
 int stkfmm_tree_get(const stkfmm_tree *tree, int *perm, int *leaf_begin, int *leaf_end, int *leaf_level,
                     int *ulist_ptr, int *rng_begin, int *rng_end);
+/* lower corner of every leaf box [3*n_leaves] (the box of a level-L leaf has edge 0.5^L) */
+int stkfmm_tree_leaf_corners(const stkfmm_tree *tree, double *corner);
 int stkfmm_tree_destroy(stkfmm_tree *tree);
+
+/* ---- equivalent / check surfaces (S2M, L2T, W- and X-list point sets) ------------------------- */
+
+/* stkfmm::surface (Lib/include/STKFMM/stkfmm_helpers.hpp:46-85): the 6(p-1)^2+2 points of the
+ * surface of the cube [c, c + alpha 0.5^depth]^3 in the reference's (PVFMM's) order, written to
+ * coord[3*count].  Returns STKFMM_OK; count = stkfmm_surface_count(p) (p >= 2). */
+int stkfmm_surface_count(int p);
+int stkfmm_surface(int p, const double *c, double alpha, int depth, double *coord);
+/* Device: the surfaces of n_boxes octree boxes at once.  Box b has lower corner d_corner[3b..3b+2]
+ * and level d_level[b]; its surface is centred on the box with half-width alpha * 0.5^(level+1)
+ * (stkfmm_surface with c = corner - 0.5^(level+1) (alpha - 1), PVFMM's placement; 1.05 / 2.95 are the
+ * equivalent / check radii, M2L/SVD_pvfmm.hpp:37-38).  d_coord [n_boxes][count][3]; bit-identical
+ * to stkfmm_surface. */
+int stkfmm_box_surfaces_device(int p, double alpha, int n_boxes, const double *d_corner, const int *d_level,
+                               double *d_coord, void *stream);
+/* k_s2m sub-kernel of a KERNEL (the kernel evaluated from sources to an upward check surface):
+ * double_layer 0 / 1.  STKFMM_ERR_PPKERNEL when the kernel has no double layer. */
+int stkfmm_get_s2m_uker(unsigned kernel, int double_layer, int *uker);
 
 /* ---- adjoints ----------------------------------------------------------------------------- */
 

@@ -151,6 +151,15 @@ def get_uker(kernel, p2p):
     return UKER(uk.value)
 
 
+def get_s2m_uker(kernel, double_layer=False):
+    """Micro-kernel k_s2m of a KERNEL: sources -> upward check surface (the BuildKernel lists of
+    LaplaceLayerKernel.hpp:315-355, StokesLayerKernel.hpp:38-88, RPYKernel.hpp:136-152,
+    StokesRegSingleLayerKernel.hpp:211-232)."""
+    uk = ctypes.c_int(-1)
+    _check(lib().stkfmm_get_s2m_uker(int(kernel), int(bool(double_layer)), ctypes.byref(uk)))
+    return UKER(uk.value)
+
+
 def uker_dims(uker):
     """(source dim, target dim) of a micro-kernel."""
     ks, kt = ctypes.c_int(), ctypes.c_int()
@@ -372,8 +381,93 @@ class Tree:
                                          ip(self.rng_end)))
             self.rng_begin = self.rng_begin[: self.n_ranges]
             self.rng_end = self.rng_end[: self.n_ranges]
+            self.leaf_corner = np.zeros((self.n_leaves, 3))
+            _check(lib().stkfmm_tree_leaf_corners(h, self.leaf_corner.ctypes.data_as(_capi.dptr)))
         finally:
             lib().stkfmm_tree_destroy(h)
+
+
+# ---- equivalent / check surfaces -----------------------------------------------------------------
+
+# PVFMM's surface radii relative to the box (M2L/SVD_pvfmm.hpp:37-38): upward equivalent and
+# downward check surfaces at 1.05, upward check and downward equivalent surfaces at 2.95
+RAD_INNER = 1.05
+RAD_OUTER = 2.95
+
+
+def surface_count(p):
+    """Points of an order-p surface: 6 (p-1)^2 + 2."""
+    n = lib().stkfmm_surface_count(int(p))
+    if n < 0:
+        raise ArgumentError("surface order p must be >= 2")
+    return n
+
+
+def surface(p, c, alpha, depth):
+    """stkfmm::surface (Lib/include/STKFMM/stkfmm_helpers.hpp:46-85) as a [count, 3] array."""
+    c = np.ascontiguousarray(c, dtype=np.float64).reshape(3)
+    out = np.zeros((surface_count(p), 3))
+    _check(lib().stkfmm_surface(int(p), c.ctypes.data_as(_capi.dptr), float(alpha), int(depth),
+                                out.ctypes.data_as(_capi.dptr)))
+    return out
+
+
+def box_surfaces_device(p, alpha, corner, level, stream=None):
+    """Surfaces of many octree boxes on the device: corner [n, 3] (float64) and level [n] (int32) CUDA
+    tensors -> [n * count, 3] tensor; box b's points are rows [b*count, (b+1)*count), centred on the
+    box with half-width alpha * 0.5^(level+1)."""
+    torch = _torch()
+    n = int(corner.shape[0])
+    _dev2d(corner, n, 3, "corner")
+    if not corner.is_contiguous() or corner.shape[1] != 3:
+        raise ArgumentError("corner: expected a contiguous [n, 3] tensor")
+    if level.dtype != torch.int32 or not level.is_cuda or not level.is_contiguous() or level.shape[0] != n:
+        raise ArgumentError("level: expected a contiguous int32 CUDA tensor of length n")
+    ns = surface_count(p)
+    out = torch.empty((n * ns, 3), dtype=torch.float64, device=corner.device)
+    _check(lib().stkfmm_box_surfaces_device(int(p), float(alpha), n, ctypes.c_void_p(corner.data_ptr()),
+                                            ctypes.c_void_p(level.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                            _stream_handle(stream)))
+    return out
+
+
+class LeafSurfaces:
+    """One order-p surface per leaf of a Tree, on the device, and the per-leaf work lists that use it.
+
+    * ``to_surface_plan``: every leaf's points (sources) evaluated at the leaf's own surface
+      (targets) -- the P2P part of PVFMM's S2M step (sources -> upward check surface, radius
+      RAD_OUTER, kernel k_s2m).
+    * ``from_surface_plan``: every leaf's surface (sources, e.g. downward equivalent densities,
+      radius RAD_OUTER) evaluated at the leaf's points -- the P2P part of L2T (kernel k_l2t).
+    Both are ordinary batched plans (one target group per leaf, level-scaled in mixed mode); the
+    surface points are ``coord`` ([n_leaves * count, 3], leaf l at rows [l*count, (l+1)*count)).
+    """
+
+    def __init__(self, tree, p, alpha=RAD_OUTER, device=None, stream=None):
+        torch = _torch()
+        device = device or torch.device("cuda", torch.cuda.current_device())
+        self.tree, self.p, self.alpha = tree, int(p), float(alpha)
+        self.count = surface_count(p)
+        corner = torch.from_numpy(tree.leaf_corner).to(device)
+        level = torch.from_numpy(tree.leaf_level.astype(np.int32)).to(device)
+        self.coord = box_surfaces_device(p, alpha, corner, level, stream)
+
+    def _surf_ranges(self):
+        nl = self.tree.n_leaves
+        b = np.arange(nl, dtype=np.int64) * self.count
+        return b, b + self.count
+
+    def to_surface_plan(self, uker, precision, tune=None):
+        t, nl = self.tree, self.tree.n_leaves
+        sb, se = self._surf_ranges()
+        return Plan(uker, precision, t.n, nl * self.count, sb, se, t.leaf_level, np.arange(nl + 1), t.leaf_begin,
+                    t.leaf_end, tune)
+
+    def from_surface_plan(self, uker, precision, tune=None):
+        t, nl = self.tree, self.tree.n_leaves
+        sb, se = self._surf_ranges()
+        return Plan(uker, precision, nl * self.count, t.n, t.leaf_begin, t.leaf_end, t.leaf_level,
+                    np.arange(nl + 1), sb, se, tune)
 
 
 # ---- batched plans ------------------------------------------------------------------------------

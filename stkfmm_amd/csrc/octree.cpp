@@ -10,6 +10,7 @@
 // Morton order, and the U-list of a leaf becomes a short list of contiguous source ranges once
 // leaves adjacent in that order are merged.
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -247,6 +248,19 @@ extern "C" int stkfmm_tree_get(const stkfmm_tree *t, int *perm, int *leaf_begin,
         std::copy(t->rng_begin.begin(), t->rng_begin.end(), rng_begin);
     if (rng_end)
         std::copy(t->rng_end.begin(), t->rng_end.end(), rng_end);
+    return STKFMM_OK;
+}
+
+extern "C" int stkfmm_tree_leaf_corners(const stkfmm_tree *t, double *corner) {
+    if (!t || !corner)
+        return STKFMM_ERR_ARG;
+    const double cell = std::ldexp(1.0, -kBits); // finest-cell edge, exact
+    const int nl = (int)t->leaf_node.size();
+    for (int l = 0; l < nl; ++l) {
+        const Node &nd = t->nodes[t->leaf_node[l]];
+        for (int k = 0; k < 3; ++k)
+            corner[3 * l + k] = nd.lo[k] * cell;
+    }
     return STKFMM_OK;
 }
 

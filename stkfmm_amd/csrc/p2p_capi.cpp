@@ -55,30 +55,43 @@ struct KernelEntry {
     unsigned kernel;
     const char *name; // ker_name given to BuildKernel
     int kdim_sl, kdim_dl, kdim_trg;
-    int uk_sl, uk_dl, uk_l2t; // -1: sub-kernel absent in the reference; -2: present but not built here
+    int uk_sl, uk_dl, uk_l2t; // -1: sub-kernel absent in the reference
+    int uk_s2m_sl, uk_s2m_dl; // k_s2m (source -> upward check surface), single and double layer
 };
 
 // kdimDL of kernels without a double layer is PVFMM's default surf_dim (PVFMM_COORD_DIM +
 // ker_dim[0]); PVFMM is not vendored, so those entries are inferred, not pinned.
-// L2T sub-kernels: LapPGrad/LapPGradGrad/PVel/Stokes use themselves (NULL in BuildKernel),
-// RPY uses stk_ulapu (RPYKernel.hpp:151), StokesRegVel uses PVFMM's Stokes velocity
-// (StokesRegSingleLayerKernel.hpp:212-214).
+// Sub-kernels from the BuildKernel lists (a NULL entry means the kernel itself):
+//   LapPGrad / LapPGradGrad: s2m laplace_p + laplace_dipolep (LaplaceLayerKernel.hpp:315-341),
+//     l2t itself;
+//   LapQPGradGrad: s2m laplace_quadp, l2t laplace_pgradgrad (LaplaceLayerKernel.hpp:344-355);
+//   Stokes: everything stokes_vel (StokesLayerKernel.hpp:38-42);
+//   RPY: s2m rpy_u, l2t stk_ulapu (RPYKernel.hpp:136-152);
+//   StokesRegVel: s2m itself, l2t PVFMM's Stokes velocity (StokesRegSingleLayerKernel.hpp:211-218);
+//   StokesRegVelOmega: s2m stokes_regftvel, l2t stokes_velomega (StokesRegSingleLayerKernel.hpp:221-232);
+//   PVel / PVelGrad / PVelLaplacian / Traction: s2m stokes_pvel + stokes_doublepvel, l2t itself
+//     (StokesLayerKernel.hpp:45-88).
 const KernelEntry kTable[] = {
-    {1, "laplace_PGrad", 1, 3, 4, STKFMM_UKER_LAPLACE_PGRAD, STKFMM_UKER_LAPLACE_DIPOLEPGRAD, STKFMM_UKER_LAPLACE_PGRAD},
+    {1, "laplace_PGrad", 1, 3, 4, STKFMM_UKER_LAPLACE_PGRAD, STKFMM_UKER_LAPLACE_DIPOLEPGRAD, STKFMM_UKER_LAPLACE_PGRAD,
+     STKFMM_UKER_LAPLACE_P, STKFMM_UKER_LAPLACE_DIPOLEP},
     {2, "laplace_PGradGrad", 1, 3, 10, STKFMM_UKER_LAPLACE_PGRADGRAD, STKFMM_UKER_LAPLACE_DIPOLEPGRADGRAD,
-     STKFMM_UKER_LAPLACE_PGRADGRAD},
-    {4, "laplace_QPGradGrad", 9, 12, 10, STKFMM_UKER_LAPLACE_QUADPGRADGRAD, -1, STKFMM_UKER_LAPLACE_PGRADGRAD},
-    {8, "stokes_vel", 3, 6, 3, STKFMM_UKER_STOKES_VEL, -1, STKFMM_UKER_STOKES_VEL},
-    {16, "rpy_ulapu", 4, 7, 6, STKFMM_UKER_RPY_ULAPU, -1, STKFMM_UKER_STK_ULAPU},
-    {32, "stokes_regvel", 4, 7, 3, STKFMM_UKER_STOKES_REGVEL, -1, STKFMM_UKER_STOKES_VEL},
-    {64, "stokes_regftvelomega", 7, 10, 6, STKFMM_UKER_STOKES_REGFTVELOMEGA, -1, STKFMM_UKER_STOKES_VELOMEGA},
-    {128, "stokes_PVel", 4, 9, 4, STKFMM_UKER_STOKES_PVEL, STKFMM_UKER_STOKES_DOUBLEPVEL, STKFMM_UKER_STOKES_PVEL},
+     STKFMM_UKER_LAPLACE_PGRADGRAD, STKFMM_UKER_LAPLACE_P, STKFMM_UKER_LAPLACE_DIPOLEP},
+    {4, "laplace_QPGradGrad", 9, 12, 10, STKFMM_UKER_LAPLACE_QUADPGRADGRAD, -1, STKFMM_UKER_LAPLACE_PGRADGRAD,
+     STKFMM_UKER_LAPLACE_QUADP, -1},
+    {8, "stokes_vel", 3, 6, 3, STKFMM_UKER_STOKES_VEL, -1, STKFMM_UKER_STOKES_VEL, STKFMM_UKER_STOKES_VEL, -1},
+    {16, "rpy_ulapu", 4, 7, 6, STKFMM_UKER_RPY_ULAPU, -1, STKFMM_UKER_STK_ULAPU, STKFMM_UKER_RPY_U, -1},
+    {32, "stokes_regvel", 4, 7, 3, STKFMM_UKER_STOKES_REGVEL, -1, STKFMM_UKER_STOKES_VEL, STKFMM_UKER_STOKES_REGVEL,
+     -1},
+    {64, "stokes_regftvelomega", 7, 10, 6, STKFMM_UKER_STOKES_REGFTVELOMEGA, -1, STKFMM_UKER_STOKES_VELOMEGA,
+     STKFMM_UKER_STOKES_REGFTVEL, -1},
+    {128, "stokes_PVel", 4, 9, 4, STKFMM_UKER_STOKES_PVEL, STKFMM_UKER_STOKES_DOUBLEPVEL, STKFMM_UKER_STOKES_PVEL,
+     STKFMM_UKER_STOKES_PVEL, STKFMM_UKER_STOKES_DOUBLEPVEL},
     {256, "stokes_PVelGrad", 4, 9, 16, STKFMM_UKER_STOKES_PVELGRAD, STKFMM_UKER_STOKES_DOUBLEPVELGRAD,
-     STKFMM_UKER_STOKES_PVELGRAD},
+     STKFMM_UKER_STOKES_PVELGRAD, STKFMM_UKER_STOKES_PVEL, STKFMM_UKER_STOKES_DOUBLEPVEL},
     {512, "stokes_PVelLaplacian", 4, 9, 7, STKFMM_UKER_STOKES_PVELLAPLACIAN, STKFMM_UKER_STOKES_DOUBLELAPLACIAN,
-     STKFMM_UKER_STOKES_PVELLAPLACIAN},
+     STKFMM_UKER_STOKES_PVELLAPLACIAN, STKFMM_UKER_STOKES_PVEL, STKFMM_UKER_STOKES_DOUBLEPVEL},
     {1024, "stokes_Traction", 4, 9, 9, STKFMM_UKER_STOKES_TRACTION, STKFMM_UKER_STOKES_DOUBLETRACTION,
-     STKFMM_UKER_STOKES_TRACTION},
+     STKFMM_UKER_STOKES_TRACTION, STKFMM_UKER_STOKES_PVEL, STKFMM_UKER_STOKES_DOUBLEPVEL},
 };
 
 const KernelEntry *find_kernel(unsigned kernel) {
@@ -126,11 +139,40 @@ extern "C" int stkfmm_get_uker(unsigned kernel, unsigned p2p, int *uker) {
         uk = -1;
     if (uk == -1)
         return set_error(STKFMM_ERR_PPKERNEL, "PPKernel %u not found for direct evaluation", p2p);
-    if (uk == -2)
-        return set_error(STKFMM_ERR_UNSUPPORTED, "kernel %s sub-kernel %u is not implemented in this library",
-                         e->name, p2p);
     if (uker)
         *uker = uk;
+    return STKFMM_OK;
+}
+
+extern "C" int stkfmm_get_s2m_uker(unsigned kernel, int double_layer, int *uker) {
+    const KernelEntry *e = find_kernel(kernel);
+    if (!e)
+        return set_error(STKFMM_ERR_KERNEL, "Error: Kernel not found.");
+    const int uk = double_layer ? e->uk_s2m_dl : e->uk_s2m_sl;
+    if (uk < 0)
+        return set_error(STKFMM_ERR_PPKERNEL, "kernel %s has no double-layer S2M sub-kernel", e->name);
+    if (uker)
+        *uker = uk;
+    return STKFMM_OK;
+}
+
+// ---- equivalent / check surfaces (stkfmm_helpers.hpp:46-85) ---------------------------------------
+
+extern "C" int stkfmm_surface_count(int p) { return p >= 2 ? stkp2p::surface_count(p) : STKFMM_ERR_ARG; }
+
+extern "C" int stkfmm_surface(int p, const double *c, double alpha, int depth, double *coord) {
+    if (p < 2 || !c || !coord || depth < 0 || depth > 60)
+        return set_error(STKFMM_ERR_ARG, "surface: need p >= 2, 0 <= depth <= 60, non-null c and coord");
+    stkp2p::surface_host(p, c, alpha, depth, coord);
+    return STKFMM_OK;
+}
+
+extern "C" int stkfmm_box_surfaces_device(int p, double alpha, int n_boxes, const double *d_corner,
+                                          const int *d_level, double *d_coord, void *stream) {
+    if (p < 2 || n_boxes < 0 || (n_boxes > 0 && (!d_corner || !d_level || !d_coord)))
+        return set_error(STKFMM_ERR_ARG, "box_surfaces: need p >= 2, n_boxes >= 0 and device arrays");
+    HIP_TRY(stkp2p::launch_box_surfaces(p, alpha, n_boxes, d_corner, d_level, d_coord, (hipStream_t)stream),
+            "box_surfaces launch");
     return STKFMM_OK;
 }
 

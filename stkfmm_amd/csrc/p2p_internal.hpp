@@ -133,6 +133,11 @@ hipError_t launch_stage_records(int uk, int prec, const StageArgs &a, int n_copi
 hipError_t launch_items_sgpr(int uk, int prec, int T, int grid, const ItemArgs &a, const double *recs,
                              int level_min, hipStream_t s);
 hipError_t items_sgpr_occupancy(int uk, int prec, int T, int *blocks_per_cu);
+// equivalent / check surfaces (surface.hip)
+__host__ __device__ inline int surface_count(int p) { return 6 * (p - 1) * (p - 1) + 2; }
+void surface_host(int p, const double *c, double alpha, int depth, double *coord);
+hipError_t launch_box_surfaces(int p, double alpha, int n_boxes, const double *corner, const int *level, double *out,
+                               hipStream_t s);
 // chunk counts: the target pass splits the sources into nchunk_t chunks, the source pass splits the
 // targets into nchunk_s chunks; partial must hold max(nchunk_t*3*n_trg, nchunk_s*(3+KS)*n_src)
 hipError_t launch_vjp(int uk, int prec, const VjpArgs &a, double *partial, int nchunk_t, int nchunk_s,
