@@ -215,6 +215,11 @@ typedef struct stkfmm_tune {
                              several lanes (default); -1: one lane per target always */
     int broadcast;        /* 0: sources staged through LDS per tile (default); 1: per-source records
                              staged in HBM once per execute, read with scalar loads */
+    int queues;           /* persistent mode: work queues, one per XCD by default (8 on MI355X, whose
+                             workgroups are dispatched to the 8 XCDs round-robin); all items of a
+                             target group go to one queue, so the group's sources stay in that XCD's
+                             L2; a wave drains its own queue, then steals from the others.
+                             1: a single cost-sorted queue */
 } stkfmm_tune;
 
 typedef struct stkfmm_plan stkfmm_plan;

@@ -222,7 +222,9 @@ struct stkfmm_plan {
     int2 *d_ranges = nullptr;
     Tile *d_tiles = nullptr;
     double *d_partial = nullptr;
-    int *d_counter = nullptr;
+    int *d_counter = nullptr; // [n_queues]
+    int *d_qptr = nullptr;    // [n_queues + 1]
+    int n_queues = 1;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     // timing ring: execute k records events ev[3*(k % kRing) + {0,1,2}] around the item and reduce
@@ -252,9 +254,13 @@ static void plan_free(stkfmm_plan *p) {
     (void)hipFree(p->d_tiles);
     (void)hipFree(p->d_partial);
     (void)hipFree(p->d_counter);
+    (void)hipFree(p->d_qptr);
     (void)hipFree(p->d_recs);
     delete p;
 }
+
+// MI355X: 8 XCDs (each with its own L2); workgroups of a launch are dispatched to them round-robin
+constexpr int kXcdsPerDevice = 8;
 
 template <class T>
 static int upload(T **dst, const std::vector<T> &src, const char *what) {
@@ -346,6 +352,7 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
     // tiles and items
     std::vector<Tile> tiles;
     std::vector<Item> items;
+    std::vector<int> item_group; // work-list group of each item
     std::vector<int2> ranges;
     std::vector<int> group_items(1, 0);
     long long pairs = 0;
@@ -384,6 +391,7 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
                         cur.cost = (int)cur_n;
                         cur.slot = (int)items.size();
                         items.push_back(cur);
+                        item_group.push_back(g);
                         cur.rng_begin = cur.rng_end = (int)ranges.size();
                         cur_n = 0;
                     }
@@ -393,6 +401,7 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
                 cur.cost = (int)cur_n;
                 cur.slot = (int)items.size();
                 items.push_back(cur);
+                item_group.push_back(g);
             }
             tl.slot_end = (int)items.size();
             tiles.push_back(tl);
@@ -400,9 +409,62 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
         if ((int)items.size() > group_items.back())
             group_items.push_back((int)items.size());
     }
-    // cost-sorted queue (largest first); ties keep work-list order.  Mode 3 keeps the groups apart.
-    if (persistent != 3)
+    // Item order.  Mode 3 keeps the groups apart in work-list order.  Mode 2, and mode 1 with a single
+    // queue: one cost-sorted queue (largest first; ties keep work-list order).  Mode 1 with several
+    // queues: groups are dealt to the queues largest-first onto the least-loaded queue (balancing the
+    // summed item costs), and each queue lists its groups largest-first with a group's items
+    // consecutive (largest first), so the waves of one XCD work through one group's sources at a
+    // time.  Item slots (and so results) do not depend on the order.
+    int n_queues = 1;
+    if (persistent == 1) {
+        n_queues = tn.queues > 0 ? tn.queues : ((n_cu % kXcdsPerDevice == 0) ? kXcdsPerDevice : 1);
+        n_queues = std::max(1, std::min({n_queues, 256, (int)items.size()}));
+    }
+    std::vector<int> q_ptr(2, 0);
+    q_ptr[1] = (int)items.size();
+    if (persistent != 3 && n_queues == 1) {
         std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) { return x.cost > y.cost; });
+    } else if (persistent == 1) {
+        std::vector<long long> gwork(n_groups, 0);
+        for (size_t i = 0; i < items.size(); ++i)
+            gwork[item_group[i]] += items[i].cost;
+        std::vector<int> gorder(n_groups);
+        for (int g = 0; g < n_groups; ++g)
+            gorder[g] = g;
+        std::stable_sort(gorder.begin(), gorder.end(), [&](int x, int y) { return gwork[x] > gwork[y]; });
+        std::vector<int> grank(n_groups), gq(n_groups, 0);
+        std::vector<long long> qload(n_queues, 0);
+        for (int r = 0; r < n_groups; ++r) {
+            const int g = gorder[r];
+            grank[g] = r;
+            int q = 0;
+            for (int k = 1; k < n_queues; ++k)
+                if (qload[k] < qload[q])
+                    q = k;
+            gq[g] = q;
+            qload[q] += gwork[g];
+        }
+        std::vector<int> idx(items.size());
+        for (size_t i = 0; i < idx.size(); ++i)
+            idx[i] = (int)i;
+        std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
+            const int gx = item_group[x], gy = item_group[y];
+            if (gq[gx] != gq[gy])
+                return gq[gx] < gq[gy];
+            if (grank[gx] != grank[gy])
+                return grank[gx] < grank[gy];
+            return items[x].cost > items[y].cost;
+        });
+        std::vector<Item> sorted(items.size());
+        q_ptr.assign(n_queues + 1, 0);
+        for (size_t i = 0; i < idx.size(); ++i) {
+            sorted[i] = items[idx[i]];
+            ++q_ptr[gq[item_group[idx[i]]] + 1];
+        }
+        for (int q = 0; q < n_queues; ++q)
+            q_ptr[q + 1] += q_ptr[q];
+        items.swap(sorted);
+    }
 
     stkfmm_plan *p = new stkfmm_plan();
     p->uk = uk;
@@ -418,6 +480,7 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
     p->n_tiles = (int)tiles.size();
     p->n_ranges = (int)ranges.size();
     p->persistent = persistent;
+    p->n_queues = n_queues;
     int wpc = tn.waves_per_cu > 0 ? std::min(tn.waves_per_cu, bpc) : bpc;
     p->grid = (persistent == 1) ? (int)std::min<long long>(p->n_items, (long long)n_cu * wpc) : p->n_items;
     if (persistent == 3)
@@ -441,7 +504,7 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
 
     int r;
     if ((r = upload(&p->d_items, items, "plan items")) || (r = upload(&p->d_ranges, ranges, "plan ranges")) ||
-        (r = upload(&p->d_tiles, tiles, "plan tiles"))) {
+        (r = upload(&p->d_tiles, tiles, "plan tiles")) || (r = upload(&p->d_qptr, q_ptr, "plan queues"))) {
         plan_free(p);
         return r;
     }
@@ -450,7 +513,7 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
     if (partial_elems)
         e = hipMalloc((void **)&p->d_partial, sizeof(double) * partial_elems);
     if (e == hipSuccess)
-        e = hipMalloc((void **)&p->d_counter, sizeof(int));
+        e = hipMalloc((void **)&p->d_counter, sizeof(int) * (size_t)p->n_queues);
     if (e == hipSuccess && p->broadcast && n_src > 0)
         e = hipMalloc((void **)&p->d_recs, sizeof(double) * (size_t)p->n_copies * n_src * p->rec_words);
     if (e != hipSuccess) {
@@ -481,6 +544,8 @@ static int plan_enqueue(stkfmm_plan *p, const stkfmm_array *sc, const stkfmm_arr
     a.ranges = p->d_ranges;
     a.n_items = p->n_items;
     a.counter = (p->persistent == 1) ? p->d_counter : nullptr;
+    a.q_ptr = p->d_qptr;
+    a.n_queues = p->n_queues;
     a.partial = p->d_partial;
     a.n_src_rec = p->n_src;
     ReduceArgs ra;
@@ -499,7 +564,7 @@ static int plan_enqueue(stkfmm_plan *p, const stkfmm_array *sc, const stkfmm_arr
         HIP_TRY(hipEventRecord(ev[0], s), "event record");
     }
     if (p->persistent == 1)
-        HIP_TRY(launch_fill_int(p->d_counter, 0, s), "counter reset launch");
+        HIP_TRY(launch_fill_int(p->d_counter, p->n_queues, 0, s), "counter reset launch");
     if (p->broadcast) {
         StageArgs sa{a.sc, a.v1, a.v2, p->n_src, p->level_min, p->d_recs};
         HIP_TRY(launch_stage_records(p->uk, p->prec, sa, p->n_copies, s), "stage records launch");

@@ -57,7 +57,10 @@ struct ItemArgs {
     const Item *items;
     const int2 *ranges;   // (src_begin, src_end)
     int n_items;
-    int *counter;         // persistent mode: next item to take; null: one workgroup per item
+    int *counter;         // persistent mode: next item of each queue [n_queues]; null: one
+                          // workgroup per item
+    const int *q_ptr;     // queue q holds items [q_ptr[q], q_ptr[q+1]) (persistent mode)
+    int n_queues;
     double *partial;      // [slot][KT][64*T]
     long long n_src_rec;  // scalar-broadcast variant: sources per staged record copy
 };
@@ -125,7 +128,7 @@ bool uker_has_vjp(int uk);
 // launchers (host); return hipSuccess or the launch error
 hipError_t launch_items(int uk, int prec, int T, int grid, const ItemArgs &a, hipStream_t s);
 hipError_t launch_reduce(const ReduceArgs &a, hipStream_t s);
-hipError_t launch_fill_int(int *p, int value, hipStream_t s);
+hipError_t launch_fill_int(int *p, int n, int value, hipStream_t s); // n <= 256
 hipError_t items_occupancy(int uk, int prec, int T, int *blocks_per_cu);
 // scalar-broadcast variant
 int record_words(int uk, int prec);

@@ -24,6 +24,7 @@
 #include <type_traits>
 
 #include "p2p_internal.hpp"
+#include "p2p_queue.hpp"
 #include "p2p_sgpr.hpp"
 #include "ukernels.hpp"
 
@@ -55,21 +56,10 @@ __global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemA
     __shared__ __attribute__((aligned(16))) float s_vf[PREC == 0 ? 1 : kSrcTile][KLP];
 
     const int lane = threadIdx.x;
-    bool first = true;
+    ItemCursor cur(a);
     for (;;) {
-        int it;
-        if (a.counter) {
-            int v = 0;
-            if (lane == 0)
-                v = atomicAdd(a.counter, 1);
-            it = __builtin_amdgcn_readfirstlane(v);
-        } else {
-            if (!first)
-                break;
-            it = blockIdx.x;
-        }
-        first = false;
-        if (it >= a.n_items)
+        const int it = next_item(a, cur, lane);
+        if (it < 0)
             break;
 
         const Item item = a.items[it];

@@ -36,7 +36,10 @@ __global__ void __launch_bounds__(256) p2p_reduce_kernel(ReduceArgs a) {
     }
 }
 
-__global__ void fill_int_kernel(int *p, int value) { *p = value; }
+__global__ void fill_int_kernel(int *p, int n, int value) {
+    if ((int)threadIdx.x < n)
+        p[threadIdx.x] = value;
+}
 
 __global__ void fill_double_kernel(double *p, long long n, double value) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -125,8 +128,10 @@ hipError_t launch_reduce(const ReduceArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_fill_int(int *p, int value, hipStream_t s) {
-    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(1), 0, s, p, value);
+hipError_t launch_fill_int(int *p, int n, int value, hipStream_t s) {
+    if (n < 1 || n > 256)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, s, p, n, value);
     return hipGetLastError();
 }
 

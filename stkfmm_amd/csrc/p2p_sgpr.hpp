@@ -18,6 +18,7 @@
 #include <type_traits>
 
 #include "p2p_internal.hpp"
+#include "p2p_queue.hpp"
 #include "ukernels.hpp"
 
 namespace stkp2p {
@@ -75,21 +76,10 @@ __global__ void __launch_bounds__(kWave) p2p_items_sgpr_kernel(ItemArgs a, const
     constexpr int KL = K::KL, KT = K::KT;
     constexpr int TT = kWave * T;
     const int lane = threadIdx.x;
-    bool first = true;
+    ItemCursor cur(a);
     for (;;) {
-        int it;
-        if (a.counter) {
-            int v = 0;
-            if (lane == 0)
-                v = atomicAdd(a.counter, 1);
-            it = __builtin_amdgcn_readfirstlane(v);
-        } else {
-            if (!first)
-                break;
-            it = blockIdx.x;
-        }
-        first = false;
-        if (it >= a.n_items)
+        const int it = next_item(a, cur, lane);
+        if (it < 0)
             break;
 
         const Item item = a.items[it];

@@ -67,7 +67,9 @@ def test_small_tree_all_leaves(uk, prec, bc):
                                   dict(targets_per_lane=2, split_tiles=-1), dict(targets_per_lane=4, split_tiles=0),
                                   dict(targets_per_lane=1, broadcast=1), dict(targets_per_lane=2, broadcast=1),
                                   dict(targets_per_lane=4, broadcast=1, persistent=2),
-                                  dict(targets_per_lane=2, broadcast=1, persistent=3)])
+                                  dict(targets_per_lane=2, broadcast=1, persistent=3),
+                                  dict(queues=1), dict(queues=3, max_src_per_item=256),
+                                  dict(queues=64, waves_per_cu=1), dict(queues=8, broadcast=1)])
 def test_tuning_knobs_do_not_change_results(tune):
     say(f"tune {tune}")
     g = np.random.default_rng(7)
@@ -77,6 +79,18 @@ def test_tuning_knobs_do_not_change_results(tune):
     out, _ = run_plan(tr, 13, 0, xs, sl, dl, tune=tune)
     l2, mx = O.max_error(out, ref)
     assert l2 < FP64_TOL and mx < FP64_TOL
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
+def test_queue_count_gives_bitwise_identical_results(prec):
+    # every item writes its own partial slot and the reduce sums slots in a fixed order, so the
+    # item order (one queue or one queue per XCD, with stealing) cannot change a bit
+    say(f"queues bitwise prec={prec}")
+    g = np.random.default_rng(8)
+    x = W.wrap(np.concatenate([g.random((8000, 3)), 0.3 + 0.01 * g.normal(size=(8000, 3))]))
+    tr, xs, sl, dl = sorted_problem(x, 150, 13, 78)
+    outs = [run_plan(tr, 13, prec, xs, sl, dl, tune=dict(queues=q, max_src_per_item=256))[0] for q in (1, 8, 5)]
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
 
 
 def test_permuted_output_and_accumulation():
