@@ -122,7 +122,9 @@ int stkfmm_p2p_eval(unsigned kernel, unsigned p2p, int precision, int nSrc, cons
                     const double *srcValue, int nTrg, const double *trgCoord, double *trgValue);
 
 /* Device-resident variant: AoS device buffers, enqueued on `stream` (hipStream_t, 0 = null stream),
- * asynchronous.  No host memory is touched. */
+ * asynchronous.  No host memory is touched.  The device entries (this one, stkfmm_wall_eval_device,
+ * stkfmm_p2p_vjp_device) keep per-device scratch (cached plans of each shape, workspaces) that
+ * calls share: calls on one device must be ordered (one stream, or synchronised), or they race. */
 int stkfmm_p2p_eval_device(unsigned kernel, unsigned p2p, int precision, int nSrc, const double *d_srcCoord,
                            const double *d_srcValue, int nTrg, const double *d_trgCoord, double *d_trgValue,
                            void *stream);
@@ -301,6 +303,20 @@ int stkfmm_box_surfaces_device(int p, double alpha, int n_boxes, const double *d
 /* k_s2m sub-kernel of a KERNEL (the kernel evaluated from sources to an upward check surface):
  * double_layer 0 / 1.  STKFMM_ERR_PPKERNEL when the kernel has no double layer. */
 int stkfmm_get_s2m_uker(unsigned kernel, int double_layer, int *uker);
+
+/* ---- wall images (StkWallFMM) ---------------------------------------------------------------- */
+
+/* The no-slip wall image system of StkWallFMM (Lib/src/StkWallFMM.cpp:154-292, kernels Stokes and
+ * RPY) evaluated with direct sums: trg_value += u over all sources and their images.  Replaces
+ * StkWallFMM::setPoints + evaluateFMM (:40-128) for a direct evaluation: the wall is the plane z = 0
+ * of the caller's coordinates (the image of (x, y, z) is (x, y, -z)); sources and targets are expected
+ * at z >= 0; no periodicity.  Stokes: source f(3) -> u(3); RPY: source [f(3), b] -> [u(3), lap u(3)].
+ * AoS arrays; other kernels: STKFMM_ERR_KERNEL ("Kernel not supported", :125-127). */
+int stkfmm_wall_evaluate(unsigned kernel, int precision, int nSL, const double *srcCoord, const double *srcValue,
+                         int nTrg, const double *trgCoord, double *trgValue);
+int stkfmm_wall_eval_device(unsigned kernel, int precision, int nSL, const double *d_srcCoord,
+                            const double *d_srcValue, int nTrg, const double *d_trgCoord, double *d_trgValue,
+                            void *stream);
 
 /* ---- adjoints ----------------------------------------------------------------------------- */
 

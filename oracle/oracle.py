@@ -69,6 +69,7 @@ def lib():
         L.oracle_direct_vjp.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp,
                                         _dp, ctypes.c_int]
         L.oracle_pair_longdouble.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+        L.oracle_wall.argtypes = [ctypes.c_uint, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -131,6 +132,18 @@ def direct(uk, src_coord, src_value, trg_coord, trg_value=None, nthreads=0, chun
     out = np.zeros(nt * kt) if trg_value is None else _f64(trg_value).reshape(-1).copy()
     fn = lib().oracle_direct_chunked if chunked else lib().oracle_direct
     assert fn(uk, ns, _d(sc), _d(sv), nt, _d(tc), _d(out), nthreads) == 0
+    return out.reshape(nt, kt)
+
+
+def wall(kernel, src_coord, src_value, trg_coord, trg_value=None, nthreads=0):
+    """StkWallFMM's image system with direct sums (kernel 8 Stokes -> 3, 16 RPY -> 6); trg_value +=."""
+    sc = _f64(src_coord).reshape(-1)
+    sv = _f64(src_value).reshape(-1)
+    tc = _f64(trg_coord).reshape(-1)
+    ns, nt = sc.size // 3, tc.size // 3
+    kt = 3 if int(kernel) == 8 else 6
+    out = np.zeros(nt * kt) if trg_value is None else _f64(trg_value).reshape(-1).copy()
+    assert lib().oracle_wall(int(kernel), ns, _d(sc), _d(sv), nt, _d(tc), _d(out), nthreads) == 0
     return out.reshape(nt, kt)
 
 

@@ -32,6 +32,7 @@
 #include <math.h>
 #include <stddef.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 #include <omp.h>
 
@@ -1175,5 +1176,119 @@ int oracle_pair_longdouble(int uk, const double *rd, const double *fd, double *u
     }
     for (int k = 0; k < uk_trg_dim[uk]; k++)
         ud[k] = (double)u[k];
+    return 0;
+}
+
+/* ---- wall images (StkWallFMM, Lib/src/StkWallFMM.cpp) --------------------------------------------
+ * StkWallFMM::evalStokes (:154-203) and ::evalRPY (:205-292) restated with every FMM evaluation
+ * replaced by the direct sum it approximates, in user coordinates: the wall is the plane z = 0, the
+ * image of a source at (x, y, z) is (x, y, -z) (setPoints, :40-80, maps z to [0.5, 1) and the image
+ * to 1 - z), and y3 / x3 of the reference are the source / target heights z.  Sources [originals;
+ * images] as in srcSLCoordInternal.  trg_value += the assembled result (evaluateFMM, :103-128).
+ * kernel: 8 (Stokes, source f(3) -> u(3)) or 16 (RPY, source [f(3), b] -> [u(3), lap u(3)]).
+ * Returns -1 for other kernels. */
+static double *wall_zeros(size_t n) {
+    double *p = (double *)calloc(n ? n : 1, sizeof(double));
+    return p;
+}
+
+int oracle_wall(unsigned kernel, int nsl, const double *src_coord, const double *src_value, int ntrg,
+                const double *trg_coord, double *trg_value, int nthreads) {
+    if (kernel != 8 && kernel != 16)
+        return -1;
+    double *c2 = wall_zeros(6 * (size_t)nsl);
+    for (int i = 0; i < nsl; i++) {
+        c2[3 * i] = c2[3 * (i + nsl)] = src_coord[3 * i];
+        c2[3 * i + 1] = c2[3 * (i + nsl) + 1] = src_coord[3 * i + 1];
+        c2[3 * i + 2] = src_coord[3 * i + 2];
+        c2[3 * (i + nsl) + 2] = -src_coord[3 * i + 2];
+    }
+    const double *cimg = c2 + 3 * (size_t)nsl;
+    if (kernel == 8) {
+        double *vS = wall_zeros(6 * (size_t)nsl), *vL1 = wall_zeros(2 * (size_t)nsl), *vD = wall_zeros(3 * (size_t)nsl),
+               *vL2 = wall_zeros(2 * (size_t)nsl);
+        double *oS = wall_zeros(3 * (size_t)ntrg), *oL1D = wall_zeros(4 * (size_t)ntrg),
+               *oL2 = wall_zeros(10 * (size_t)ntrg);
+        for (int i = 0; i < nsl; i++) {
+            const double *f = src_value + 3 * (size_t)i;
+            const double y3 = src_coord[3 * i + 2];
+            vS[3 * i] = f[0];
+            vS[3 * i + 1] = f[1];
+            vS[3 * (i + nsl)] = -f[0];
+            vS[3 * (i + nsl) + 1] = -f[1];
+            vL1[i] = -0.5 * f[2];
+            vL1[i + nsl] = 0.5 * f[2];
+            vD[3 * i] = -y3 * f[0];
+            vD[3 * i + 1] = -y3 * f[1];
+            vD[3 * i + 2] = y3 * f[2];
+            vL2[i] = f[2] * y3;
+            vL2[i + nsl] = -vL2[i];
+        }
+        oracle_direct(UK_STOKES_VEL, 2 * nsl, c2, vS, ntrg, trg_coord, oS, nthreads);
+        oracle_direct(UK_LAPLACE_PGRAD, 2 * nsl, c2, vL1, ntrg, trg_coord, oL1D, nthreads);
+        oracle_direct(UK_LAPLACE_DIPOLEPGRAD, nsl, cimg, vD, ntrg, trg_coord, oL1D, nthreads);
+        oracle_direct(UK_LAPLACE_PGRADGRAD, 2 * nsl, c2, vL2, ntrg, trg_coord, oL2, nthreads);
+        for (int i = 0; i < ntrg; i++) {
+            const double x3 = trg_coord[3 * i + 2];
+            double u[3];
+            for (int j = 0; j < 3; j++)
+                u[j] = oS[3 * i + j] + 0.5 * oL2[10 * i + j + 1] + x3 * oL1D[4 * i + j + 1];
+            u[2] -= oL1D[4 * i];
+            for (int j = 0; j < 3; j++)
+                trg_value[3 * i + j] += u[j];
+        }
+        free(vS); free(vL1); free(vD); free(vL2); free(oS); free(oL1D); free(oL2);
+    } else {
+        double *vR = wall_zeros(8 * (size_t)nsl), *vLS = wall_zeros(2 * (size_t)nsl), *vLD = wall_zeros(3 * (size_t)nsl),
+               *vLSZ = wall_zeros(2 * (size_t)nsl), *vLDZ = wall_zeros(6 * (size_t)nsl), *vQ = wall_zeros(9 * (size_t)nsl);
+        double *oR = wall_zeros(6 * (size_t)ntrg), *oSD = wall_zeros(10 * (size_t)ntrg),
+               *oSDZ = wall_zeros(4 * (size_t)ntrg), *oQ = wall_zeros(10 * (size_t)ntrg);
+        for (int i = 0; i < nsl; i++) {
+            const double *f = src_value + 4 * (size_t)i;
+            const double y3 = src_coord[3 * i + 2];
+            const double b = f[3], b2 = b * b, twob2 = 2 * b * b;
+            vR[4 * i] = f[0];
+            vR[4 * i + 1] = f[1];
+            vR[4 * i + 3] = b;
+            vR[4 * (i + nsl)] = -f[0];
+            vR[4 * (i + nsl) + 1] = -f[1];
+            vR[4 * (i + nsl) + 3] = b;
+            vLS[i] = f[2] * (-0.5);
+            vLS[i + nsl] = -f[2] * (-0.5);
+            vLD[3 * i] = -y3 * f[0];
+            vLD[3 * i + 1] = -y3 * f[1];
+            vLD[3 * i + 2] = y3 * f[2];
+            vLSZ[i] = y3 * f[2] * 0.5;
+            vLSZ[i + nsl] = -y3 * f[2] * 0.5;
+            vLDZ[3 * i + 2] = b2 * f[2] * (1. / 6.);
+            vLDZ[3 * (i + nsl) + 2] = b2 * f[2] * (1. / 6.);
+            vQ[9 * i] = twob2 * f[2] * (1. / 6.);
+            vQ[9 * i + 4] = twob2 * f[2] * (1. / 6.);
+            vQ[9 * i + 6] = twob2 * f[0] * (1. / 6.);
+            vQ[9 * i + 7] = twob2 * f[1] * (1. / 6.);
+        }
+        oracle_direct(UK_RPY_ULAPU, 2 * nsl, c2, vR, ntrg, trg_coord, oR, nthreads);
+        oracle_direct(UK_LAPLACE_PGRADGRAD, 2 * nsl, c2, vLS, ntrg, trg_coord, oSD, nthreads);
+        oracle_direct(UK_LAPLACE_DIPOLEPGRADGRAD, nsl, cimg, vLD, ntrg, trg_coord, oSD, nthreads);
+        oracle_direct(UK_LAPLACE_PGRAD, 2 * nsl, c2, vLSZ, ntrg, trg_coord, oSDZ, nthreads);
+        oracle_direct(UK_LAPLACE_DIPOLEPGRAD, 2 * nsl, c2, vLDZ, ntrg, trg_coord, oSDZ, nthreads);
+        oracle_direct(UK_LAPLACE_QUADPGRADGRAD, nsl, cimg, vQ, ntrg, trg_coord, oQ, nthreads);
+        for (int i = 0; i < ntrg; i++) {
+            const double x3 = trg_coord[3 * i + 2];
+            const double *R = oR + 6 * (size_t)i, *SD = oSD + 10 * (size_t)i, *SDZ = oSDZ + 4 * (size_t)i,
+                         *Q = oQ + 10 * (size_t)i;
+            double u[6];
+            u[0] = R[0] + SDZ[1] + x3 * SD[1] + x3 * Q[1];
+            u[1] = R[1] + SDZ[2] + x3 * SD[2] + x3 * Q[2];
+            u[2] = R[2] + SDZ[3] + x3 * SD[3] + x3 * Q[3] - SD[0] - Q[0];
+            u[3] = R[3] + 2 * SD[6] + 2 * Q[6];
+            u[4] = R[4] + 2 * SD[8] + 2 * Q[8];
+            u[5] = R[5] + 2 * SD[9] + 2 * Q[9];
+            for (int j = 0; j < 6; j++)
+                trg_value[6 * i + j] += u[j];
+        }
+        free(vR); free(vLS); free(vLD); free(vLSZ); free(vLDZ); free(vQ); free(oR); free(oSD); free(oSDZ); free(oQ);
+    }
+    free(c2);
     return 0;
 }

@@ -246,6 +246,29 @@ def ker_eval(uker, r_src, v_src, r_trg, v_trg, dof=1):
     return v_trg
 
 
+def _wall_dims(kernel):
+    if int(kernel) == KERNEL.Stokes:
+        return 3, 3
+    if int(kernel) == KERNEL.RPY:
+        return 4, 6
+    raise KernelNotFound("Kernel not supported")
+
+
+def wall_evaluate(kernel, src_coord, src_value, trg_coord, trg_value, precision=PREC.FP64):
+    """StkWallFMM's no-slip wall image system (Lib/src/StkWallFMM.cpp:154-292) by direct sums, host
+    arrays: trg_value += u.  The wall is z = 0; KERNEL.Stokes (f -> u) or KERNEL.RPY ([f, b] ->
+    [u, lap u])."""
+    ks, kt = _wall_dims(kernel)
+    ns = np.asarray(src_coord).reshape(-1, 3).shape[0]
+    nt = np.asarray(trg_coord).reshape(-1, 3).shape[0]
+    sc = _host_arr(src_coord, ns, 3, "src_coord")
+    sv = _host_arr(src_value, ns, ks, "src_value")
+    tc = _host_arr(trg_coord, nt, 3, "trg_coord")
+    tv = _host_arr(trg_value, nt, kt, "trg_value", writable=True)
+    _check(lib().stkfmm_wall_evaluate(int(kernel), int(precision), ns, sc, sv, nt, tc, tv))
+    return trg_value
+
+
 # ---- device (torch) entries ----------------------------------------------------------------------
 
 def _torch():
@@ -347,6 +370,22 @@ def vjp_device(uker, src_coord, src_value, trg_coord, w, precision=PREC.FP64, ne
 
 
 # ---- octree / U-list ---------------------------------------------------------------------------
+
+def wall_device(kernel, src_coord, src_value, trg_coord, trg_value, precision=PREC.FP64, stream=None):
+    """``wall_evaluate`` on contiguous AoS float64 CUDA tensors; trg_value += u (asynchronous)."""
+    ks, kt = _wall_dims(kernel)
+    ns, nt = int(src_coord.shape[0]), int(trg_coord.shape[0])
+    for t, n, k, name in ((src_coord, ns, 3, "src_coord"), (src_value, ns, ks, "src_value"),
+                          (trg_coord, nt, 3, "trg_coord"), (trg_value, nt, kt, "trg_value")):
+        _dev2d(t, n, k, name)
+        if not t.is_contiguous() or t.shape[1] != k:
+            raise ArgumentError(f"{name}: expected a contiguous [{n}, {k}] tensor")
+    _check(lib().stkfmm_wall_eval_device(int(kernel), int(precision), ns, ctypes.c_void_p(src_coord.data_ptr()),
+                                         ctypes.c_void_p(src_value.data_ptr()), nt,
+                                         ctypes.c_void_p(trg_coord.data_ptr()), ctypes.c_void_p(trg_value.data_ptr()),
+                                         _stream_handle(stream)))
+    return trg_value
+
 
 class Tree:
     """Adaptive octree + U-list over points in [0,1)^3 (host).

@@ -46,7 +46,8 @@ EXPORTS = [
     "stkfmm_plan_launch_graph", "stkfmm_plan_stats", "stkfmm_plan_set_timing", "stkfmm_plan_kernel_ms",
     "stkfmm_plan_destroy", "stkfmm_tree_build", "stkfmm_tree_info", "stkfmm_tree_get", "stkfmm_tree_destroy",
     "stkfmm_p2p_vjp_device", "stkfmm_device_info", "stkfmm_tree_leaf_corners", "stkfmm_surface_count",
-    "stkfmm_surface", "stkfmm_box_surfaces_device", "stkfmm_get_s2m_uker",
+    "stkfmm_surface", "stkfmm_box_surfaces_device", "stkfmm_get_s2m_uker", "stkfmm_wall_evaluate",
+    "stkfmm_wall_eval_device",
 ] + ["stkfmm_ker_" + n for n in KER_T_NAMES]
 
 
@@ -86,6 +87,9 @@ def _declare(L):
     L.stkfmm_surface.argtypes = [c_int, dptr, ctypes.c_double, c_int, dptr]
     L.stkfmm_box_surfaces_device.argtypes = [c_int, ctypes.c_double, c_int, c_void_p, c_void_p, c_void_p, c_void_p]
     L.stkfmm_get_s2m_uker.argtypes = [c_uint, c_int, iptr]
+    L.stkfmm_wall_evaluate.argtypes = [c_uint, c_int, c_int, dptr, dptr, c_int, dptr, dptr]
+    L.stkfmm_wall_eval_device.argtypes = [c_uint, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                                          c_void_p]
     L.stkfmm_p2p_vjp_device.argtypes = [c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                                         c_void_p, c_void_p, c_void_p, c_void_p]
     L.stkfmm_device_info.argtypes = [iptr, iptr, c_char_p, c_int]

@@ -748,9 +748,9 @@ int direct_plan(int uk, int prec, int n_src, int n_trg, stkfmm_plan **out) {
 // grow-only device buffers for the host-pointer entries
 struct Workspace {
     int device = -1;
-    // slots 0-3: host-pointer direct sums; slot 4: adjoint partial sums
-    double *buf[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    size_t cap[5] = {0, 0, 0, 0, 0};
+    // slots 0-3: host-pointer direct sums; slot 4: adjoint partial sums; slot 5: wall images
+    double *buf[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    size_t cap[6] = {0, 0, 0, 0, 0, 0};
 };
 std::vector<Workspace> g_ws;
 
@@ -853,6 +853,118 @@ extern "C" int stkfmm_p2p_eval_device(unsigned kernel, unsigned p2p, int precisi
     std::lock_guard<std::mutex> lk(g_mu);
     return eval_device_uk(uk, precision, nSrc, d_srcCoord, d_srcValue, nTrg, d_trgCoord, d_trgValue,
                           (hipStream_t)stream);
+}
+
+// ---- wall images (StkWallFMM, Lib/src/StkWallFMM.cpp:154-292) ----------------------------------------
+
+namespace {
+// the constituent direct sums of StkWallFMM::evalStokes / evalRPY, with the layouts of wall.hip
+int wall_device(unsigned kernel, int prec, int n, const double *sc, const double *sv, int m, const double *tc,
+                double *tv, hipStream_t s) {
+    const size_t N = (size_t)n, M = (size_t)m;
+    double *w = nullptr;
+    int rc;
+    if (kernel == STKFMM_KERNEL_Stokes) {
+        if ((rc = ws_get(5, 6 * N + 6 * N + 2 * N + 3 * N + 2 * N + 17 * M, &w)))
+            return rc;
+        WallStokes L;
+        L.c2 = w;
+        L.vS = L.c2 + 6 * N;
+        L.vL1 = L.vS + 6 * N;
+        L.vD = L.vL1 + 2 * N;
+        L.vL2 = L.vD + 3 * N;
+        L.oS = L.vL2 + 2 * N;
+        L.oL1D = L.oS + 3 * M;
+        L.oL2 = L.oL1D + 4 * M;
+        HIP_TRY(launch_wall_prep_stokes(n, sc, sv, L, s), "wall prep launch");
+        HIP_TRY(launch_wall_zero(L.oS, (long long)(17 * M), s), "wall zero launch");
+        const double *cimg = L.c2 + 3 * N;
+        if ((rc = eval_device_uk(STKFMM_UKER_STOKES_VEL, prec, 2 * n, L.c2, L.vS, m, tc, L.oS, s)) ||
+            (rc = eval_device_uk(STKFMM_UKER_LAPLACE_PGRAD, prec, 2 * n, L.c2, L.vL1, m, tc, L.oL1D, s)) ||
+            (rc = eval_device_uk(STKFMM_UKER_LAPLACE_DIPOLEPGRAD, prec, n, cimg, L.vD, m, tc, L.oL1D, s)) ||
+            (rc = eval_device_uk(STKFMM_UKER_LAPLACE_PGRADGRAD, prec, 2 * n, L.c2, L.vL2, m, tc, L.oL2, s)))
+            return rc;
+        HIP_TRY(launch_wall_assemble_stokes(m, tc, L, tv, s), "wall assemble launch");
+        return STKFMM_OK;
+    }
+    if ((rc = ws_get(5, 6 * N + 8 * N + 2 * N + 3 * N + 2 * N + 6 * N + 9 * N + 30 * M, &w)))
+        return rc;
+    WallRpy L;
+    L.c2 = w;
+    L.vR = L.c2 + 6 * N;
+    L.vLS = L.vR + 8 * N;
+    L.vLD = L.vLS + 2 * N;
+    L.vLSZ = L.vLD + 3 * N;
+    L.vLDZ = L.vLSZ + 2 * N;
+    L.vQ = L.vLDZ + 6 * N;
+    L.oR = L.vQ + 9 * N;
+    L.oSD = L.oR + 6 * M;
+    L.oSDZ = L.oSD + 10 * M;
+    L.oQ = L.oSDZ + 4 * M;
+    HIP_TRY(launch_wall_prep_rpy(n, sc, sv, L, s), "wall prep launch");
+    HIP_TRY(launch_wall_zero(L.oR, (long long)(30 * M), s), "wall zero launch");
+    const double *cimg = L.c2 + 3 * N;
+    if ((rc = eval_device_uk(STKFMM_UKER_RPY_ULAPU, prec, 2 * n, L.c2, L.vR, m, tc, L.oR, s)) ||
+        (rc = eval_device_uk(STKFMM_UKER_LAPLACE_PGRADGRAD, prec, 2 * n, L.c2, L.vLS, m, tc, L.oSD, s)) ||
+        (rc = eval_device_uk(STKFMM_UKER_LAPLACE_DIPOLEPGRADGRAD, prec, n, cimg, L.vLD, m, tc, L.oSD, s)) ||
+        (rc = eval_device_uk(STKFMM_UKER_LAPLACE_PGRAD, prec, 2 * n, L.c2, L.vLSZ, m, tc, L.oSDZ, s)) ||
+        (rc = eval_device_uk(STKFMM_UKER_LAPLACE_DIPOLEPGRAD, prec, 2 * n, L.c2, L.vLDZ, m, tc, L.oSDZ, s)) ||
+        (rc = eval_device_uk(STKFMM_UKER_LAPLACE_QUADPGRADGRAD, prec, n, cimg, L.vQ, m, tc, L.oQ, s)))
+        return rc;
+    HIP_TRY(launch_wall_assemble_rpy(m, tc, L, tv, s), "wall assemble launch");
+    return STKFMM_OK;
+}
+
+int wall_check(unsigned kernel, int precision, int n, int m) {
+    if (kernel != STKFMM_KERNEL_Stokes && kernel != STKFMM_KERNEL_RPY)
+        return set_error(STKFMM_ERR_KERNEL, "Kernel not supported"); // StkWallFMM.cpp:125-127
+    if (precision != STKFMM_PREC_FP64 && precision != STKFMM_PREC_MIXED)
+        return set_error(STKFMM_ERR_ARG, "unknown precision %d", precision);
+    if (n < 0 || m < 0 || n > (1 << 29))
+        return set_error(STKFMM_ERR_ARG, "bad point count");
+    return STKFMM_OK;
+}
+} // namespace
+
+extern "C" int stkfmm_wall_eval_device(unsigned kernel, int precision, int nSL, const double *d_srcCoord,
+                                       const double *d_srcValue, int nTrg, const double *d_trgCoord,
+                                       double *d_trgValue, void *stream) {
+    int rc = wall_check(kernel, precision, nSL, nTrg);
+    if (rc)
+        return rc;
+    if (nSL == 0 || nTrg == 0)
+        return STKFMM_OK;
+    if (!d_srcCoord || !d_srcValue || !d_trgCoord || !d_trgValue)
+        return set_error(STKFMM_ERR_ARG, "NULL buffer");
+    std::lock_guard<std::mutex> lk(g_mu);
+    return wall_device(kernel, precision, nSL, d_srcCoord, d_srcValue, nTrg, d_trgCoord, d_trgValue,
+                       (hipStream_t)stream);
+}
+
+extern "C" int stkfmm_wall_evaluate(unsigned kernel, int precision, int nSL, const double *srcCoord,
+                                    const double *srcValue, int nTrg, const double *trgCoord, double *trgValue) {
+    int rc = wall_check(kernel, precision, nSL, nTrg);
+    if (rc)
+        return rc;
+    if (nSL == 0 || nTrg == 0)
+        return STKFMM_OK;
+    if (!srcCoord || !srcValue || !trgCoord || !trgValue)
+        return set_error(STKFMM_ERR_ARG, "NULL buffer");
+    const size_t ks = kernel == STKFMM_KERNEL_Stokes ? 3 : 4, kt = kernel == STKFMM_KERNEL_Stokes ? 3 : 6;
+    std::lock_guard<std::mutex> lk(g_mu);
+    double *d_sc, *d_sv, *d_tc, *d_tv;
+    if ((rc = ws_get(0, 3 * (size_t)nSL, &d_sc)) || (rc = ws_get(1, ks * nSL, &d_sv)) ||
+        (rc = ws_get(2, 3 * (size_t)nTrg, &d_tc)) || (rc = ws_get(3, kt * nTrg, &d_tv)))
+        return rc;
+    HIP_TRY(hipMemcpy(d_sc, srcCoord, sizeof(double) * 3 * nSL, hipMemcpyHostToDevice), "copy srcCoord");
+    HIP_TRY(hipMemcpy(d_sv, srcValue, sizeof(double) * ks * nSL, hipMemcpyHostToDevice), "copy srcValue");
+    HIP_TRY(hipMemcpy(d_tc, trgCoord, sizeof(double) * 3 * nTrg, hipMemcpyHostToDevice), "copy trgCoord");
+    HIP_TRY(hipMemcpy(d_tv, trgValue, sizeof(double) * kt * nTrg, hipMemcpyHostToDevice), "copy trgValue");
+    rc = wall_device(kernel, precision, nSL, d_sc, d_sv, nTrg, d_tc, d_tv, nullptr);
+    if (rc)
+        return rc;
+    HIP_TRY(hipMemcpy(trgValue, d_tv, sizeof(double) * kt * nTrg, hipMemcpyDeviceToHost), "copy trgValue back");
+    return STKFMM_OK;
 }
 
 extern "C" int stkfmm_ker_eval(int uker, const double *r_src, int src_cnt, const double *v_src, int dof,

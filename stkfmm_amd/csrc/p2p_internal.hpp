@@ -141,6 +141,21 @@ __host__ __device__ inline int surface_count(int p) { return 6 * (p - 1) * (p - 
 void surface_host(int p, const double *c, double alpha, int depth, double *coord);
 hipError_t launch_box_surfaces(int p, double alpha, int n_boxes, const double *corner, const int *level, double *out,
                                hipStream_t s);
+// wall image system (wall.hip): device views of the workspace of one evaluation.  n sources
+// (2n with images: [originals; images]), m targets; v* are source values, o* accumulated outputs.
+struct WallStokes {
+    double *c2, *vS, *vL1, *vD, *vL2; // [2n*3], [2n*3], [2n], [n*3] (at the images), [2n]
+    double *oS, *oL1D, *oL2;          // [m*3], [m*4], [m*10]
+};
+struct WallRpy {
+    double *c2, *vR, *vLS, *vLD, *vLSZ, *vLDZ, *vQ; // [2n*3],[2n*4],[2n],[n*3],[2n],[2n*3],[n*9]
+    double *oR, *oSD, *oSDZ, *oQ;                   // [m*6], [m*10], [m*4], [m*10]
+};
+hipError_t launch_wall_prep_stokes(int n, const double *sc, const double *sv, const WallStokes &w, hipStream_t s);
+hipError_t launch_wall_prep_rpy(int n, const double *sc, const double *sv, const WallRpy &w, hipStream_t s);
+hipError_t launch_wall_zero(double *p, long long n, hipStream_t s);
+hipError_t launch_wall_assemble_stokes(int m, const double *tc, const WallStokes &w, double *out, hipStream_t s);
+hipError_t launch_wall_assemble_rpy(int m, const double *tc, const WallRpy &w, double *out, hipStream_t s);
 // chunk counts: the target pass splits the sources into nchunk_t chunks, the source pass splits the
 // targets into nchunk_s chunks; partial must hold max(nchunk_t*3*n_trg, nchunk_s*(3+KS)*n_src)
 hipError_t launch_vjp(int uk, int prec, const VjpArgs &a, double *partial, int nchunk_t, int nchunk_s,
