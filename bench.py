@@ -243,9 +243,18 @@ def lapgrad_direct(args, torch, S, W):
     d_src, d_q, d_trg = (torch.from_numpy(a).to(dev) for a in (src, q, trg))
     out = torch.zeros((len(trg), 4), dtype=torch.float64, device=dev)
     uker = S.get_uker(S.KERNEL.LapPGrad, S.PPKERNEL.SLS2T)
-    plan = S.Plan(uker, S.PREC.FP64, len(src), len(trg), [0], [len(trg)], None, [0, 1], [0], [len(src)])
+    work = dict(n_src=len(src), n_trg=len(trg), grp_trg_begin=[0], grp_trg_end=[len(trg)], grp_level=None,
+                grp_rng_ptr=[0, 1], rng_begin=[0], rng_end=[len(src)])
     stream = torch.cuda.Stream()
     kw = dict(src_coord=d_src, src_value=d_q, trg_coord=d_trg, trg_value=out)
+    # a single 8192 x 8192 sum is small for 256 CUs: tune the item size down as well
+    from stkfmm_amd.autotune import autotune
+    cands = [dict(targets_per_lane=t, max_src_per_item=m, persistent=p, broadcast=b)
+             for t in (1, 2, 4) for m in (256, 512, 1024, 2048) for p in (1, 2) for b in (0, 1)]
+    scratch = torch.zeros_like(out)
+    tune, _ = autotune(work, uker, S.PREC.FP64, dict(kw, trg_value=scratch, stream=stream), candidates=cands)
+    del scratch
+    plan = S.Plan(uker, S.PREC.FP64, tune=tune, **work)
     steps = max(args.steps, 50)
     wall, item_ms, reduce_ms, count = Timed(torch, plan, kw, stream)(steps, args.warmup)
     pairs = len(src) * len(trg)
@@ -280,8 +289,9 @@ def lapgrad_direct(args, torch, S, W):
     line("Gpair-interactions/s Laplace PGrad direct sum (evaluateKernel), FP64", pairs * steps / wall / 1e9,
          "Gpair/s", args, 1e3 * wall / steps, "f64",
          {"workload": "lapgrad-direct-8192x8192", "pairs": pairs, "precision": "fp64"},
-         host_api_pcie_inclusive_gpair_s=pairs / host_s / 1e9,
-         roofline=roofline("lapgrad", "fp64", pairs, item_ms, f"p2p_items_kernel<{int(uker)},FP64,T=2>"),
+         host_api_pcie_inclusive_gpair_s=pairs / host_s / 1e9, tune=tune,
+         roofline=roofline("lapgrad", "fp64", pairs, item_ms,
+                           f"p2p_items_kernel<{int(uker)},FP64,T={tune.get('targets_per_lane', 2)}>"),
          cpu_baseline=cpu, accuracy_vs_oracle=acc, kernel_ms={"item": item_ms, "reduce": reduce_ms})
 
 

@@ -35,6 +35,16 @@ namespace stkp2p {
 #ifndef STKP2P_MIN_WAVES_PER_EU
 #define STKP2P_MIN_WAVES_PER_EU 0
 #endif
+// STKP2P_MIXED_HILO (build option, default 1): in mixed mode each level-scaled coordinate is held
+// as an FP32 pair hi + lo (hi = fl32(x), lo = fl32(x - hi), ~48 bits together) and the displacement
+// is formed in FP32 as (t_hi - s_hi) + (t_lo - s_lo): the hi difference is exact whenever the two
+// coordinates are within a factor 2 of each other (Sterbenz), the lo difference is tiny, so the
+// result is the FP64 displacement rounded once to FP32 -- as with an FP64 subtraction and a
+// conversion, but in three FP32 operations per axis instead of two FP64-rate ones.  0: FP64
+// coordinates, FP64 subtraction then conversion.
+#ifndef STKP2P_MIXED_HILO
+#define STKP2P_MIXED_HILO 1
+#endif
 #if STKP2P_MIN_WAVES_PER_EU > 0
 #define STKP2P_ITEM_ATTR __attribute__((amdgpu_waves_per_eu(STKP2P_MIN_WAVES_PER_EU, 8)))
 #else
@@ -49,9 +59,11 @@ __global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemA
 
     // FP64: values as [KL][tile] (two consecutive sources per 16-byte read);
     // mixed: FP32 values as 16-byte aligned rows [tile][KLP] (one source per few 16-byte reads,
-    // all at immediate offsets from one base address)
-    constexpr int KLP = ((KL + 3) / 4) * 4;
-    __shared__ double s_x[kSrcTile], s_y[kSrcTile], s_z[kSrcTile];
+    // all at immediate offsets from one base address), led by the hi/lo coordinates with HILO
+    constexpr bool HILO = (PREC != 0) && (STKP2P_MIXED_HILO != 0);
+    constexpr int KOFF = HILO ? 6 : 0;
+    constexpr int KLP = ((KL + KOFF + 3) / 4) * 4;
+    __shared__ double s_x[HILO ? 1 : kSrcTile], s_y[HILO ? 1 : kSrcTile], s_z[HILO ? 1 : kSrcTile];
     __shared__ double s_vd[PREC == 0 ? KL : 1][kSrcTile];
     __shared__ __attribute__((aligned(16))) float s_vf[PREC == 0 ? 1 : kSrcTile][KLP];
 
@@ -79,6 +91,18 @@ __global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemA
             tx[t] = a.tc.p[idx * a.tc.s] * lam;
             ty[t] = a.tc.p[idx * a.tc.s + a.tc.fs] * lam;
             tz[t] = a.tc.p[idx * a.tc.s + 2 * a.tc.fs] * lam;
+        }
+        float thx[T], thy[T], thz[T], tlx[T], tly[T], tlz[T];
+        if constexpr (HILO) {
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                thx[t] = (float)tx[t];
+                thy[t] = (float)ty[t];
+                thz[t] = (float)tz[t];
+                tlx[t] = (float)(tx[t] - (double)thx[t]);
+                tly[t] = (float)(ty[t] - (double)thy[t]);
+                tlz[t] = (float)(tz[t] - (double)thz[t]);
+            }
         }
         double acc[T][KT];
 #pragma unroll
@@ -113,15 +137,25 @@ __global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemA
                     }
                     double vst[KL];
                     K::stage(vin, vst);
-                    s_x[lane] = x;
-                    s_y[lane] = y;
-                    s_z[lane] = z;
+                    if constexpr (HILO) {
+                        const float hx = (float)x, hy = (float)y, hz = (float)z;
+                        s_vf[lane][0] = hx;
+                        s_vf[lane][1] = hy;
+                        s_vf[lane][2] = hz;
+                        s_vf[lane][3] = (float)(x - (double)hx);
+                        s_vf[lane][4] = (float)(y - (double)hy);
+                        s_vf[lane][5] = (float)(z - (double)hz);
+                    } else {
+                        s_x[lane] = x;
+                        s_y[lane] = y;
+                        s_z[lane] = z;
+                    }
 #pragma unroll
                     for (int k = 0; k < KL; ++k) {
                         if (PREC == 0)
                             s_vd[k][lane] = vst[k];
                         else
-                            s_vf[lane][k] = (float)vst[k];
+                            s_vf[lane][KOFF + k] = (float)vst[k];
                     }
                 }
                 __syncthreads();
@@ -154,15 +188,25 @@ __global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemA
                                     part[t][k] = 0.0f;
 #pragma unroll 2
                             for (int j = j0 + j_first; j < j1; j += j_step) {
-                                const double sx = s_x[j], sy = s_y[j], sz = s_z[j];
                                 float f[KL];
 #pragma unroll
                                 for (int k = 0; k < KL; ++k)
-                                    f[k] = s_vf[j][k];
+                                    f[k] = s_vf[j][KOFF + k];
+                                if constexpr (HILO) {
+                                    const float hx = s_vf[j][0], hy = s_vf[j][1], hz = s_vf[j][2];
+                                    const float lx = s_vf[j][3], ly = s_vf[j][4], lz = s_vf[j][5];
 #pragma unroll
-                                for (int t = 0; t < T; ++t)
-                                    K::template eval<float>(part[t], (float)(tx[t] - sx), (float)(ty[t] - sy),
-                                                            (float)(tz[t] - sz), f);
+                                    for (int t = 0; t < T; ++t)
+                                        K::template eval<float>(part[t], (thx[t] - hx) + (tlx[t] - lx),
+                                                                (thy[t] - hy) + (tly[t] - ly),
+                                                                (thz[t] - hz) + (tlz[t] - lz), f);
+                                } else {
+                                    const double sx = s_x[j], sy = s_y[j], sz = s_z[j];
+#pragma unroll
+                                    for (int t = 0; t < T; ++t)
+                                        K::template eval<float>(part[t], (float)(tx[t] - sx), (float)(ty[t] - sy),
+                                                                (float)(tz[t] - sz), f);
+                                }
                             }
 #pragma unroll
                             for (int t = 0; t < T; ++t)
