@@ -202,9 +202,9 @@ __device__ inline void dl_pvel_gradp(R *u, const DlFrame<R> &a, R trd) {
 }
 
 // stokes_doublepvelgrad, StokesDoubleLayerKernel.hpp:71-142; D row-major (9) -> 16.
-//   du_i/dx_j (:131-141, rinv7 already times -3):
+//   du_i/dx_j (:131-141, rinv7 already times -3; the bracket is of degree 4 in r):
 //     = -3 s^7 (-5 (r.D.r) r_i r_j + r^2 r_i (D r + D^T r)_j + r^2 (r.D.r) d_ij)
-//     = 3 s^4 (5 Qh n_i n_j - n_i c_j - Qh d_ij);   post() 3
+//     = 3 s^3 (5 Qh n_i n_j - n_i c_j - Qh d_ij);   post() 3
 struct UkStokesDoublePVelGrad {
     static constexpr int KS = 9, KS1 = 9, KL = 7, KT = 16;
     static constexpr unsigned SRC_SCALE_MASK = 0x1FFu;
@@ -215,14 +215,13 @@ struct UkStokesDoublePVelGrad {
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *g) {
         const DlFrame<R> a(dx, dy, dz, g);
         dl_pvel_gradp(u, a, g[6]);
-        const R s2 = a.s * a.s;
-        const R s4 = s2 * s2;
+        const R s3 = a.s * a.s * a.s;
         const R q5 = R(5.0) * a.qh;
-        const R dq = s4 * a.qh;
+        const R dq = s3 * a.qh;
         R e[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            e[j] = s4 * (q5 * a.n[j] - a.c[j]);
+            e[j] = s3 * (q5 * a.n[j] - a.c[j]);
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -251,10 +250,10 @@ struct UkStokesDoubleLaplacian {
 
 // stokes_doubletraction, StokesDoubleLayerKernel.hpp:220-272 (ScaleFactor -3/(8 pi), :217):
 //   np = facp r^2 s^7 (r^2 trD - 3 r.D.r) = facp s^3 (trD - 3 Qh)    (facp = 0.66666666666666)
-//   tv_ij = s^7 (-5 (r.D.r) r_i r_j + r^2 r_i (D r + D^T r)_j + r^2 (r.D.r) d_ij)
-//         = s^4 (-5 Qh n_i n_j + n_i c_j + Qh d_ij)
-//   sigma_ij = d_ij np + tv_ij + tv_ji = d_ij (np + 2 s^4 Qh) + n_i h_j + n_j h_i,
-//   h_j = s^4 (c_j - 5 Qh n_j)
+//   tv_ij = s^7 (-5 (r.D.r) r_i r_j + r^2 r_i (D r + D^T r)_j + r^2 (r.D.r) d_ij)   (degree 4)
+//         = s^3 (-5 Qh n_i n_j + n_i c_j + Qh d_ij)
+//   sigma_ij = d_ij np + tv_ij + tv_ji = d_ij (np + 2 s^3 Qh) + n_i h_j + n_j h_i,
+//   h_j = s^3 (c_j - 5 Qh n_j)
 struct UkStokesDoubleTraction {
     static constexpr int KS = 9, KS1 = 9, KL = 7, KT = 9;
     static constexpr unsigned SRC_SCALE_MASK = 0x1FFu;
@@ -264,16 +263,14 @@ struct UkStokesDoubleTraction {
     template <class R>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *g) {
         const DlFrame<R> a(dx, dy, dz, g);
-        const R s2 = a.s * a.s;
-        const R s3 = s2 * a.s;
-        const R s4 = s2 * s2;
+        const R s3 = a.s * a.s * a.s;
         const R np = R(0.66666666666666) * (s3 * (g[6] - R(3.0) * a.qh));
-        const R diag = np + R(2.0) * (s4 * a.qh);
+        const R diag = np + R(2.0) * (s3 * a.qh);
         const R q5 = R(5.0) * a.qh;
         R h[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            h[j] = s4 * (a.c[j] - q5 * a.n[j]);
+            h[j] = s3 * (a.c[j] - q5 * a.n[j]);
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll

@@ -1,0 +1,69 @@
+// ukernel_host.cpp -- test harness: the library's per-pair micro-kernels (stage, eval, post) compiled
+// for the CPU, one pair at a time, so tests/test_ukernels_host.py can check their algebra against the
+// oracle in FP64 and FP32 without a GPU.  Built by stkfmm_amd/csrc/Makefile with the host shim in
+// tests/cxx/shim.  Not part of the library.
+#include "../../stkfmm_amd/csrc/ukernels_family.hpp"
+
+using namespace stkp2p;
+
+namespace {
+template <class K, class R>
+void pair(const double *r, const double *fin, double scale, double *u) {
+    double vst[K::KL];
+    double vin[K::KS];
+    for (int k = 0; k < K::KS; ++k)
+        vin[k] = fin[k];
+    K::stage(vin, vst);
+    R f[K::KL];
+    for (int k = 0; k < K::KL; ++k)
+        f[k] = (R)vst[k];
+    R acc[K::KT];
+    for (int k = 0; k < K::KT; ++k)
+        acc[k] = R(0);
+    K::template eval<R>(acc, (R)r[0], (R)r[1], (R)r[2], f);
+    for (int k = 0; k < K::KT; ++k)
+        u[k] = (double)acc[k] * K::post(k) * scale;
+}
+
+template <class K>
+int run(int prec, const double *r, const double *f, double *u) {
+    if (prec == 0)
+        pair<K, double>(r, f, 1.0, u);
+    else
+        pair<K, float>(r, f, 1.0, u);
+    return K::KT;
+}
+} // namespace
+
+// u = the micro-kernel's sum for one pair (no ScaleFactor), prec 0 = FP64, 1 = FP32 arithmetic;
+// returns the number of outputs or -1 for an unknown id
+extern "C" int ukernel_host_pair(int uk, int prec, const double *r, const double *f, double *u) {
+    switch (uk) {
+    case 0: return run<UkLaplaceP>(prec, r, f, u);
+    case 1: return run<UkLaplaceDipoleP>(prec, r, f, u);
+    case 2: return run<UkLaplacePGrad>(prec, r, f, u);
+    case 3: return run<UkLaplaceDipolePGrad>(prec, r, f, u);
+    case 4: return run<UkLaplacePGradGrad>(prec, r, f, u);
+    case 5: return run<UkLaplaceDipolePGradGrad>(prec, r, f, u);
+    case 6: return run<UkStokesVel>(prec, r, f, u);
+    case 7: return run<UkStokesPVel>(prec, r, f, u);
+    case 8: return run<UkStokesDoublePVel>(prec, r, f, u);
+    case 9: return run<UkStokesRegVel>(prec, r, f, u);
+    case 10: return run<UkRpyU>(prec, r, f, u);
+    case 11: return run<UkRpyULapU>(prec, r, f, u);
+    case 12: return run<UkStkULapU>(prec, r, f, u);
+    case 13: return run<UkStokesPVelFused>(prec, r, f, u);
+    case 14: return run<UkStokesPVelGrad>(prec, r, f, u);
+    case 15: return run<UkStokesDoublePVelGrad>(prec, r, f, u);
+    case 16: return run<UkStokesPVelLaplacian>(prec, r, f, u);
+    case 17: return run<UkStokesDoubleLaplacian>(prec, r, f, u);
+    case 18: return run<UkStokesTraction>(prec, r, f, u);
+    case 19: return run<UkStokesDoubleTraction>(prec, r, f, u);
+    case 20: return run<UkStokesRegFTVelOmega>(prec, r, f, u);
+    case 21: return run<UkStokesVelOmega>(prec, r, f, u);
+    case 22: return run<UkLaplaceQuadPGradGrad>(prec, r, f, u);
+    case 23: return run<UkLaplaceQuadP>(prec, r, f, u);
+    case 24: return run<UkStokesRegFTVel>(prec, r, f, u);
+    default: return -1;
+    }
+}

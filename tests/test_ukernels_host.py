@@ -1,0 +1,78 @@
+"""The library's per-pair micro-kernels (stkfmm_amd/csrc/ukernels*.hpp: stage, eval, post), compiled for
+the CPU by tests/cxx/ukernel_host.cpp, against the oracle -- the algebra of every GPU micro-kernel
+(its rewrites of the reference expression) checked without a GPU.
+
+FP64: to 1e-12 of each output's largest entry, the parity bar of the FP64 mode (the rewritten
+expressions round differently from the reference's order; for outputs that cancel, such as
+f_i r^2 - 3 r_i (f.r), that difference reaches a few 1e-13).  FP32: the same pipeline in single precision, to a bound of FP32 rounding amplified by
+the kernel's cancellation, on pairs at ordinary and at close range (the unit-direction forms of the
+r^-7 / r^-9 kernels must not overflow)."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.conftest import RADIUS_UKERS, ROOT
+
+LIB = os.path.join(ROOT, "tests", "cxx", "libukernel_host.so")
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def lib():
+    if not os.path.exists(LIB):
+        pytest.skip("tests/cxx/libukernel_host.so not built (make -C stkfmm_amd/csrc)")
+    L = ctypes.CDLL(LIB)
+    L.ukernel_host_pair.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
+    return L
+
+
+def host_pair(uk, prec, r, f):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    u = np.zeros(32)
+    kt = lib().ukernel_host_pair(uk, prec, r.ctypes.data_as(_dp), f.ctypes.data_as(_dp), u.ctypes.data_as(_dp))
+    assert kt == O.trg_dim(uk)
+    return u[:kt]
+
+
+def values(uk, rng):
+    f = rng.uniform(-1, 1, O.src_dim(uk))
+    if uk in RADIUS_UKERS:
+        f[-1] = abs(f[-1]) * 0.01 + 1e-3
+    return f
+
+
+@pytest.mark.parametrize("uk", range(25))
+def test_fp64_pairs_match_oracle(uk):
+    rng = np.random.default_rng(400 + uk)
+    for _ in range(200):
+        r = rng.uniform(-1, 1, 3) * 10 ** rng.uniform(-3, 1)
+        f = values(uk, rng)
+        a, b = host_pair(uk, 0, r, f), O.pair(uk, r, f)
+        np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-12 * np.abs(b).max())
+
+
+@pytest.mark.parametrize("uk", range(25))
+def test_coincident_pair_is_zero(uk):
+    f = np.ones(O.src_dim(uk))
+    if uk in (9, 20, 24):
+        f[-1] = 0.0
+    for prec in (0, 1):
+        assert np.all(host_pair(uk, prec, np.zeros(3), f) == 0.0)
+
+
+@pytest.mark.parametrize("uk", range(25))
+def test_fp32_pairs_match_oracle(uk):
+    # per-pair bound: each output is a sum of a few terms; FP32 rounding of the inputs and of each
+    # operation gives a relative error of a few 1e-7 of the largest term.  Close range included
+    # (r down to 1e-4 in the level-scaled frame).
+    rng = np.random.default_rng(500 + uk)
+    for _ in range(300):
+        r = rng.uniform(-1, 1, 3) * 10 ** rng.uniform(-4, 0.5)
+        f = values(uk, rng)
+        a, b = host_pair(uk, 1, r, f), O.pair(uk, r, f)
+        assert np.all(np.isfinite(a))
+        scale = np.abs(b).max()
+        assert np.abs(a - b).max() <= 2e-5 * scale, (r, np.abs(a - b).max() / scale)
