@@ -22,28 +22,48 @@
 
 namespace stkp2p {
 
-// single-layer Stokes source [fx, fy, fz, TrD]; m = f.n - TrD s (the reference's (f.r - TrD) / r)
+// single-layer Stokes source [fx, fy, fz, TrD].  The reference keeps one running coefficient,
+// c = f.r, and updates it in place: c - TrD for u, (c - TrD) + TrD for grad p / lap u, and
+// ((c - TrD) + TrD) - TrD for grad u (StokesSingleLayerKernel.hpp:76-98, 139-145).  When
+// |f.r| << |TrD| (close pairs) each round trip through TrD costs f.r relative accuracy, and the
+// reference's results carry that; FP64 reproduces the sequence so that it agrees with the reference to
+// rounding.  FP32 uses the exact expressions (there the round trip would cost ~1e-7 |TrD| / |f.r|).
+// Unit-direction forms: with s = 1/r, n = r s,  c s^k r-powers become  (c s) s^(k-1).
 template <class R>
 struct SlFrame {
-    R s, nx, ny, nz, fn, m;
+    R s, nx, ny, nz;
+    R c0, c1, c2, c3; // (f.r) s, (f.r - TrD) s, and the two later states of the coefficient, times s
     __device__ SlFrame(R dx, R dy, R dz, const R *f) {
         const R r2 = dx * dx + dy * dy + dz * dz;
         s = rsqrt_masked(r2);
         nx = dx * s;
         ny = dy * s;
         nz = dz * s;
-        fn = f[0] * nx + f[1] * ny + f[2] * nz;
-        m = fn - f[3] * s;
+        if constexpr (sizeof(R) == 8) {
+            const R cc = f[0] * dx + f[1] * dy + f[2] * dz;
+            const R a1 = cc - f[3];
+            const R a2 = a1 + f[3];
+            const R a3 = a2 - f[3];
+            c0 = cc * s;
+            c1 = a1 * s;
+            c2 = a2 * s;
+            c3 = a3 * s;
+        } else {
+            c0 = f[0] * nx + f[1] * ny + f[2] * nz;
+            c1 = c0 - f[3] * s;
+            c2 = c0;
+            c3 = c1;
+        }
     }
 };
 
 // stokes_pvelgrad, StokesSingleLayerKernel.hpp:61-110; source [fx, fy, fz, TrD] ->
-// [p, u(3), grad p(3), grad u(9) row-major du_i/dx_j].  With r^2 s^2 = 1:
-//   p = 2 (f.r) s^3 = 2 fn s^2
-//   u_i = s^3 (r^2 f_i + (f.r - TrD) r_i) = s f_i + (m s) n_i
-//   dp/dx_i = 2 s^5 (r^2 f_i - 3 r_i (f.r)) = 2 s^3 (f_i - 3 n_i fn)
+// [p, u(3), grad p(3), grad u(9) row-major du_i/dx_j].  With r^2 s^2 = 1 (c0..c3 of SlFrame):
+//   p = 2 (f.r) s^3 = 2 c0 s^2
+//   u_i = s^3 (r^2 f_i + (f.r - TrD) r_i) = s f_i + (c1 s) n_i
+//   dp/dx_i = 2 s^5 (r^2 f_i - 3 r_i (f.r)) = 2 s^3 (f_i - 3 n_i c2)
 //   du_i/dx_j = s^5 ((r^2 d_ij - 3 r_i r_j)(f.r - TrD) + r^2 (r_i f_j - r_j f_i))
-//             = s^2 (d_ij m - 3 n_i n_j m + n_i f_j - n_j f_i)
+//             = s^2 (d_ij c3 - 3 n_i n_j c3 + n_i f_j - n_j f_i)
 // post() applies the 2 of p and grad p.
 struct UkStokesPVelGrad {
     static constexpr int KS = 4, KS1 = 4, KL = 4, KT = 16;
@@ -59,16 +79,16 @@ struct UkStokesPVelGrad {
         const R n[3] = {a.nx, a.ny, a.nz};
         const R s2 = a.s * a.s;
         const R s3 = s2 * a.s;
-        const R ms = a.m * a.s;
-        const R g = R(3.0) * a.fn;
-        u[0] += a.fn * s2;
+        const R ms = a.c1 * a.s;
+        const R g = R(3.0) * a.c2;
+        u[0] += a.c0 * s2;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             u[1 + i] += f[i] * a.s;
             u[1 + i] += n[i] * ms;
             u[4 + i] += s3 * (f[i] - n[i] * g);
         }
-        const R M = a.m * s2;
+        const R M = a.c3 * s2;
         R p3[3], ns[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
@@ -90,7 +110,7 @@ struct UkStokesPVelGrad {
 };
 
 // stokes_pvellaplacian, StokesSingleLayerKernel.hpp:125-154: [p, u(3), lap u(3)];
-//   lap u_i = 2 s^5 (f_i r^2 - 3 r_i (f.r)) = 2 s^3 (f_i - 3 n_i fn)   (= grad p of PVelGrad)
+//   lap u_i = 2 s^5 (f_i r^2 - 3 r_i (f.r)) = 2 s^3 (f_i - 3 n_i c2)   (= grad p of PVelGrad)
 struct UkStokesPVelLaplacian {
     static constexpr int KS = 4, KS1 = 4, KL = 4, KT = 7;
     static constexpr unsigned SRC_SCALE_MASK = 0x8u;
@@ -105,9 +125,9 @@ struct UkStokesPVelLaplacian {
         const R n[3] = {a.nx, a.ny, a.nz};
         const R s2 = a.s * a.s;
         const R s3 = s2 * a.s;
-        const R ms = a.m * a.s;
-        const R g = R(3.0) * a.fn;
-        u[0] += a.fn * s2;
+        const R ms = a.c1 * a.s;
+        const R g = R(3.0) * a.c2;
+        u[0] += a.c0 * s2;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             u[1 + i] += f[i] * a.s;
@@ -118,7 +138,7 @@ struct UkStokesPVelLaplacian {
 };
 
 // stokes_traction, StokesSingleLayerKernel.hpp:169-194 (ScaleFactor -3/(4 pi), :166): stress row-major
-//   sigma_ij = r_i r_j (f.r - TrD) s^5 + d_ij TrD r^2 s^5 / 3 = n_i n_j m s^2 + d_ij (TrD/3) s^3
+//   sigma_ij = r_i r_j (f.r - TrD) s^5 + d_ij TrD r^2 s^5 / 3 = n_i n_j c1 s^2 + d_ij (TrD/3) s^3
 // Staged: fx, fy, fz, TrD, TrD * invthree (the reference's invthree = 0.3333333333333333333).
 struct UkStokesTraction {
     static constexpr int KS = 4, KS1 = 4, KL = 5, KT = 9;
@@ -134,7 +154,7 @@ struct UkStokesTraction {
         const SlFrame<R> a(dx, dy, dz, f);
         const R n[3] = {a.nx, a.ny, a.nz};
         const R s2 = a.s * a.s;
-        const R mm = a.m * s2;
+        const R mm = a.c1 * s2;
         const R diag = f[4] * (s2 * a.s);
 #pragma unroll
         for (int i = 0; i < 3; ++i) {

@@ -76,3 +76,16 @@ def test_fp32_pairs_match_oracle(uk):
         assert np.all(np.isfinite(a))
         scale = np.abs(b).max()
         assert np.abs(a - b).max() <= 2e-5 * scale, (r, np.abs(a - b).max() / scale)
+
+
+@pytest.mark.parametrize("uk", range(25))
+def test_fp64_close_pairs_match_reference_rounding(uk):
+    # r down to 1e-8: where the reference's in-place updates of its coefficient lose accuracy
+    # (e.g. (f.r - TrD) + TrD in the Stokes gradient kernels), the FP64 kernels follow the same
+    # sequence and stay within 1e-12 of it
+    rng = np.random.default_rng(600 + uk)
+    for _ in range(200):
+        r = rng.uniform(-1, 1, 3) * 10 ** rng.uniform(-8, -3)
+        f = values(uk, rng)
+        a, b = host_pair(uk, 0, r, f), O.pair(uk, r, f)
+        np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-12 * np.abs(b).max())
