@@ -304,6 +304,25 @@ int stkfmm_box_surfaces_device(int p, double alpha, int n_boxes, const double *d
  * double_layer 0 / 1.  STKFMM_ERR_PPKERNEL when the kernel has no double layer. */
 int stkfmm_get_s2m_uker(unsigned kernel, int double_layer, int *uker);
 
+/* ---- box mapping and value scaling (the frame an FMM evaluation works in) --------------------- */
+
+/* STKFMM::scaleCoord + wrapCoord (Lib/src/STKFMM.cpp:113-150): coord <- (coord - origin) * scaleFactor,
+ * then x - floor(x) on the periodic axes (pbc = PAXIS: 0 none, 1 x, 2 xy, 3 xyz).  In place. */
+int stkfmm_scale_coord(int n, double *coord, const double *origin, double scaleFactor, unsigned pbc);
+/* FMMData::scaleSrc (Lib/src/FMMData.cpp:285-317): srcDLValue *= scaleFactor; the single-layer
+ * components with the dimension of a length (TrD of the PVel family, eps of StokesRegVel, b of RPY,
+ * torque and eps of StokesRegVelOmega) *= scaleFactor.  In place; kernel = KERNEL bit. */
+int stkfmm_scale_src(unsigned kernel, int nSL, double *srcSLValue, int nDL, double *srcDLValue, double scaleFactor);
+/* FMMData::scaleTrg (Lib/src/FMMData.cpp:319-470): every output component times the power of
+ * scaleFactor of its dimension (e.g. PVel: p * sF^2, u * sF).  In place. */
+int stkfmm_scale_trg(unsigned kernel, int nTrg, double *trgValue, double scaleFactor);
+/* The same on device arrays (origin stays a host pointer), enqueued on stream; bit-identical. */
+int stkfmm_scale_coord_device(int n, double *d_coord, const double *origin, double scaleFactor, unsigned pbc,
+                              void *stream);
+int stkfmm_scale_src_device(unsigned kernel, int nSL, double *d_srcSLValue, int nDL, double *d_srcDLValue,
+                            double scaleFactor, void *stream);
+int stkfmm_scale_trg_device(unsigned kernel, int nTrg, double *d_trgValue, double scaleFactor, void *stream);
+
 /* ---- wall images (StkWallFMM) ---------------------------------------------------------------- */
 
 /* The no-slip wall image system of StkWallFMM (Lib/src/StkWallFMM.cpp:154-292, kernels Stokes and

@@ -246,6 +246,77 @@ def ker_eval(uker, r_src, v_src, r_trg, v_trg, dof=1):
     return v_trg
 
 
+# ---- box mapping and value scaling (STKFMM::scaleCoord/wrapCoord, FMMData::scaleSrc/scaleTrg) ----
+
+def _np_or_torch(a):
+    torch = None
+    try:
+        import torch  # noqa: F811
+    except ImportError:  # pragma: no cover
+        pass
+    return torch is not None and isinstance(a, torch.Tensor)
+
+
+def scale_coord(coord, origin, scale_factor, pbc=PAXIS.NONE, stream=None):
+    """In place: coord <- (coord - origin) * scale_factor, wrapped to [0,1) on the periodic axes
+    (Lib/src/STKFMM.cpp:113-150).  coord: [n, 3] float64 numpy array or contiguous CUDA tensor."""
+    o = np.ascontiguousarray(origin, dtype=np.float64).reshape(3)
+    if _np_or_torch(coord):
+        _dev2d(coord, coord.shape[0], 3, "coord")
+        if not coord.is_contiguous():
+            raise ArgumentError("coord: expected a contiguous tensor")
+        _check(lib().stkfmm_scale_coord_device(int(coord.shape[0]), ctypes.c_void_p(coord.data_ptr()),
+                                               o.ctypes.data_as(_capi.dptr), float(scale_factor), int(pbc),
+                                               _stream_handle(stream)))
+        return coord
+    n = np.asarray(coord).reshape(-1, 3).shape[0]
+    p = _host_arr(coord, n, 3, "coord", writable=True)
+    _check(lib().stkfmm_scale_coord(n, p, o.ctypes.data_as(_capi.dptr), float(scale_factor), int(pbc)))
+    return coord
+
+
+def scale_src(kernel, src_sl_value, src_dl_value, scale_factor, stream=None):
+    """In place: FMMData::scaleSrc (Lib/src/FMMData.cpp:285-317).  Arrays [n, kdim] (numpy or CUDA);
+    either may be None."""
+    ks, kd, _ = get_kernel_dimension(kernel)
+    sl, dl = src_sl_value, src_dl_value
+    if (sl is not None and _np_or_torch(sl)) or (dl is not None and _np_or_torch(dl)):
+        nsl = 0 if sl is None else int(sl.shape[0])
+        ndl = 0 if dl is None else int(dl.shape[0])
+        for t, n, k, name in ((sl, nsl, ks, "src_sl_value"), (dl, ndl, kd, "src_dl_value")):
+            if t is not None:
+                _dev2d(t, n, k, name)
+                if not t.is_contiguous() or t.shape[1] != k:
+                    raise ArgumentError(f"{name}: expected a contiguous [{n}, {k}] tensor")
+        _check(lib().stkfmm_scale_src_device(int(kernel), nsl, ctypes.c_void_p(0 if sl is None else sl.data_ptr()),
+                                             ndl, ctypes.c_void_p(0 if dl is None else dl.data_ptr()),
+                                             float(scale_factor), _stream_handle(stream)))
+        return sl, dl
+    nsl = 0 if sl is None else np.asarray(sl).size // ks
+    ndl = 0 if dl is None else np.asarray(dl).size // kd
+    ps = None if sl is None else _host_arr(sl, nsl, ks, "src_sl_value", writable=True)
+    pd = None if dl is None else _host_arr(dl, ndl, kd, "src_dl_value", writable=True)
+    _check(lib().stkfmm_scale_src(int(kernel), nsl, ps, ndl, pd, float(scale_factor)))
+    return sl, dl
+
+
+def scale_trg(kernel, trg_value, scale_factor, stream=None):
+    """In place: FMMData::scaleTrg (Lib/src/FMMData.cpp:319-470).  trg_value [n, kdimTrg]."""
+    kt = get_kernel_dimension(kernel)[2]
+    if _np_or_torch(trg_value):
+        n = int(trg_value.shape[0])
+        _dev2d(trg_value, n, kt, "trg_value")
+        if not trg_value.is_contiguous() or trg_value.shape[1] != kt:
+            raise ArgumentError(f"trg_value: expected a contiguous [{n}, {kt}] tensor")
+        _check(lib().stkfmm_scale_trg_device(int(kernel), n, ctypes.c_void_p(trg_value.data_ptr()),
+                                             float(scale_factor), _stream_handle(stream)))
+        return trg_value
+    n = np.asarray(trg_value).size // kt
+    _check(lib().stkfmm_scale_trg(int(kernel), n, _host_arr(trg_value, n, kt, "trg_value", writable=True),
+                                  float(scale_factor)))
+    return trg_value
+
+
 def _wall_dims(kernel):
     if int(kernel) == KERNEL.Stokes:
         return 3, 3

@@ -47,7 +47,8 @@ EXPORTS = [
     "stkfmm_plan_destroy", "stkfmm_tree_build", "stkfmm_tree_info", "stkfmm_tree_get", "stkfmm_tree_destroy",
     "stkfmm_p2p_vjp_device", "stkfmm_device_info", "stkfmm_tree_leaf_corners", "stkfmm_surface_count",
     "stkfmm_surface", "stkfmm_box_surfaces_device", "stkfmm_get_s2m_uker", "stkfmm_wall_evaluate",
-    "stkfmm_wall_eval_device",
+    "stkfmm_wall_eval_device", "stkfmm_scale_coord", "stkfmm_scale_src", "stkfmm_scale_trg",
+    "stkfmm_scale_coord_device", "stkfmm_scale_src_device", "stkfmm_scale_trg_device",
 ] + ["stkfmm_ker_" + n for n in KER_T_NAMES]
 
 
@@ -87,6 +88,12 @@ def _declare(L):
     L.stkfmm_surface.argtypes = [c_int, dptr, ctypes.c_double, c_int, dptr]
     L.stkfmm_box_surfaces_device.argtypes = [c_int, ctypes.c_double, c_int, c_void_p, c_void_p, c_void_p, c_void_p]
     L.stkfmm_get_s2m_uker.argtypes = [c_uint, c_int, iptr]
+    L.stkfmm_scale_coord.argtypes = [c_int, dptr, dptr, ctypes.c_double, c_uint]
+    L.stkfmm_scale_src.argtypes = [c_uint, c_int, dptr, c_int, dptr, ctypes.c_double]
+    L.stkfmm_scale_trg.argtypes = [c_uint, c_int, dptr, ctypes.c_double]
+    L.stkfmm_scale_coord_device.argtypes = [c_int, c_void_p, dptr, ctypes.c_double, c_uint, c_void_p]
+    L.stkfmm_scale_src_device.argtypes = [c_uint, c_int, c_void_p, c_int, c_void_p, ctypes.c_double, c_void_p]
+    L.stkfmm_scale_trg_device.argtypes = [c_uint, c_int, c_void_p, ctypes.c_double, c_void_p]
     L.stkfmm_wall_evaluate.argtypes = [c_uint, c_int, c_int, dptr, dptr, c_int, dptr, dptr]
     L.stkfmm_wall_eval_device.argtypes = [c_uint, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                                           c_void_p]

@@ -35,6 +35,9 @@ static int set_error(int code, const char *fmt, ...) {
     return code;
 }
 
+// for the entries defined in other translation units (box.hip)
+int stkp2p::report_error(int code, const char *msg) { return set_error(code, "%s", msg); }
+
 static int hip_error(hipError_t e, const char *what) {
     return set_error(STKFMM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
 }

@@ -118,6 +118,9 @@ constexpr int kNumUkerTotal = 25;  // 14..24 in p2p_family.hip
 const UkerOps *uker_ops(int uk);
 const UkerOps *uker_ops_family(int uk);
 
+// sets the message stkfmm_last_error() returns; returns code
+int report_error(int code, const char *msg);
+
 // kernel-table queries (host)
 int uker_ks(int uk);
 int uker_ks1(int uk);
