@@ -284,6 +284,16 @@ int stkfmm_tree_get(const stkfmm_tree *tree, int *perm, int *leaf_begin, int *le
                     int *ulist_ptr, int *rng_begin, int *rng_end);
 /* lower corner of every leaf box [3*n_leaves] (the box of a level-L leaf has edge 0.5^L) */
 int stkfmm_tree_leaf_corners(const stkfmm_tree *tree, double *corner);
+/* All nodes (non-empty boxes, root = 0): counts, and per node its lower corner [3*n], level, sorted
+ * point range [begin, end), leaf id (-1: internal) and parent (-1: root).  Any output may be NULL. */
+int stkfmm_tree_node_info(const stkfmm_tree *tree, int *n_nodes, int *n_w);
+int stkfmm_tree_get_nodes(const stkfmm_tree *tree, double *corner, int *level, int *begin, int *end, int *leaf_id,
+                          int *parent);
+/* W- and X-lists of the adaptive tree (PVFMM's M2T and S2L near-field lists, not vendored; the
+ * kernel-independent FMM's definitions): W(leaf B) = nodes deeper than B that do not touch B but whose
+ * parent does; X(node C) = the leaves whose W-list holds C.  CSR: w_ptr [n_leaves+1] into w_node
+ * [n_w] (node indices); x_ptr [n_nodes+1] into x_leaf [n_w] (leaf ids). */
+int stkfmm_tree_get_wx(const stkfmm_tree *tree, int *w_ptr, int *w_node, int *x_ptr, int *x_leaf);
 int stkfmm_tree_destroy(stkfmm_tree *tree);
 
 /* ---- equivalent / check surfaces (S2M, L2T, W- and X-list point sets) ------------------------- */
@@ -303,6 +313,10 @@ int stkfmm_box_surfaces_device(int p, double alpha, int n_boxes, const double *d
 /* k_s2m sub-kernel of a KERNEL (the kernel evaluated from sources to an upward check surface):
  * double_layer 0 / 1.  STKFMM_ERR_PPKERNEL when the kernel has no double layer. */
 int stkfmm_get_s2m_uker(unsigned kernel, int double_layer, int *uker);
+/* k_s2l (sources -> downward check surface, the X-list kernel) and k_m2t (upward equivalent
+ * densities -> targets, the W-list kernel; single layer) of a KERNEL, from the BuildKernel lists. */
+int stkfmm_get_s2l_uker(unsigned kernel, int double_layer, int *uker);
+int stkfmm_get_m2t_uker(unsigned kernel, int *uker);
 
 /* ---- box mapping and value scaling (the frame an FMM evaluation works in) --------------------- */
 

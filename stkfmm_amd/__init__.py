@@ -160,6 +160,20 @@ def get_s2m_uker(kernel, double_layer=False):
     return UKER(uk.value)
 
 
+def get_s2l_uker(kernel, double_layer=False):
+    """Micro-kernel k_s2l of a KERNEL: sources -> downward check surface (the X-list kernel)."""
+    uk = ctypes.c_int(-1)
+    _check(lib().stkfmm_get_s2l_uker(int(kernel), int(bool(double_layer)), ctypes.byref(uk)))
+    return UKER(uk.value)
+
+
+def get_m2t_uker(kernel):
+    """Micro-kernel k_m2t of a KERNEL: upward equivalent densities -> targets (the W-list kernel)."""
+    uk = ctypes.c_int(-1)
+    _check(lib().stkfmm_get_m2t_uker(int(kernel), ctypes.byref(uk)))
+    return UKER(uk.value)
+
+
 def uker_dims(uker):
     """(source dim, target dim) of a micro-kernel."""
     ks, kt = ctypes.c_int(), ctypes.c_int()
@@ -462,7 +476,10 @@ class Tree:
     """Adaptive octree + U-list over points in [0,1)^3 (host).
 
     Attributes (numpy int32): perm (sorted -> original index), leaf_begin, leaf_end, leaf_level,
-    ulist_ptr, rng_begin, rng_end; plus n_leaves, n_ranges, max_level, pairs.
+    ulist_ptr, rng_begin, rng_end; plus n_leaves, n_ranges, max_level, pairs; leaf_corner [n_leaves, 3].
+    All nodes (non-empty boxes, root 0): node_corner, node_level, node_begin, node_end, node_leaf
+    (-1 internal), node_parent; W-lists w_ptr/w_node (per leaf, node indices) and X-lists
+    x_ptr/x_leaf (per node, leaf ids).
     """
 
     def __init__(self, coord, leaf_cap):
@@ -493,6 +510,22 @@ class Tree:
             self.rng_end = self.rng_end[: self.n_ranges]
             self.leaf_corner = np.zeros((self.n_leaves, 3))
             _check(lib().stkfmm_tree_leaf_corners(h, self.leaf_corner.ctypes.data_as(_capi.dptr)))
+            nn, nw = ctypes.c_int(), ctypes.c_int()
+            _check(lib().stkfmm_tree_node_info(h, ctypes.byref(nn), ctypes.byref(nw)))
+            self.n_nodes = nn.value
+            self.node_corner = np.zeros((self.n_nodes, 3))
+            self.node_level, self.node_begin, self.node_end, self.node_leaf, self.node_parent = (
+                np.zeros(self.n_nodes, dtype=np.int32) for _ in range(5))
+            _check(lib().stkfmm_tree_get_nodes(h, self.node_corner.ctypes.data_as(_capi.dptr), ip(self.node_level),
+                                               ip(self.node_begin), ip(self.node_end), ip(self.node_leaf),
+                                               ip(self.node_parent)))
+            self.w_ptr = np.zeros(self.n_leaves + 1, dtype=np.int32)
+            self.w_node = np.zeros(max(nw.value, 1), dtype=np.int32)
+            self.x_ptr = np.zeros(self.n_nodes + 1, dtype=np.int32)
+            self.x_leaf = np.zeros(max(nw.value, 1), dtype=np.int32)
+            _check(lib().stkfmm_tree_get_wx(h, ip(self.w_ptr), ip(self.w_node), ip(self.x_ptr), ip(self.x_leaf)))
+            self.w_node = self.w_node[: nw.value]
+            self.x_leaf = self.x_leaf[: nw.value]
         finally:
             lib().stkfmm_tree_destroy(h)
 
@@ -539,6 +572,44 @@ def box_surfaces_device(p, alpha, corner, level, stream=None):
                                             ctypes.c_void_p(level.data_ptr()), ctypes.c_void_p(out.data_ptr()),
                                             _stream_handle(stream)))
     return out
+
+
+class NodeSurfaces:
+    """One order-p surface per node of a Tree (all non-empty boxes), on the device, and the W- and
+    X-list work lists that use them (the M2T and S2L evaluations of PVFMM's near field):
+
+    * ``w_plan``: for every leaf B, the upward equivalent surfaces (radius RAD_INNER) of the nodes in
+      W(B), carrying densities, evaluated at B's points (kernel k_m2t, ``get_m2t_uker``);
+    * ``x_plan``: for every node C with a non-empty X(C), the points of the leaves in X(C) evaluated
+      at C's downward check surface (radius RAD_INNER; kernel k_s2l, ``get_s2l_uker``).
+    ``coord`` is [n_nodes * count, 3], node c at rows [c*count, (c+1)*count).
+    """
+
+    def __init__(self, tree, p, alpha=RAD_INNER, device=None, stream=None):
+        torch = _torch()
+        device = device or torch.device("cuda", torch.cuda.current_device())
+        self.tree, self.p, self.alpha = tree, int(p), float(alpha)
+        self.count = surface_count(p)
+        corner = torch.from_numpy(tree.node_corner).to(device)
+        level = torch.from_numpy(tree.node_level.astype(np.int32)).to(device)
+        self.coord = box_surfaces_device(p, alpha, corner, level, stream)
+
+    def w_plan(self, uker, precision, tune=None):
+        t = self.tree
+        ns = self.count
+        rb = t.w_node.astype(np.int64) * ns
+        return Plan(uker, precision, t.n_nodes * ns, t.n, t.leaf_begin, t.leaf_end, t.leaf_level, t.w_ptr, rb,
+                    rb + ns, tune)
+
+    def x_plan(self, uker, precision, tune=None):
+        t = self.tree
+        ns = self.count
+        nodes = np.nonzero(np.diff(t.x_ptr) > 0)[0]
+        gb = nodes.astype(np.int64) * ns
+        ptr = np.concatenate([[0], np.cumsum(np.diff(t.x_ptr)[nodes])])
+        leaves = t.x_leaf
+        return Plan(uker, precision, t.n, t.n_nodes * ns, gb, gb + ns, t.node_level[nodes], ptr,
+                    t.leaf_begin[leaves], t.leaf_end[leaves], tune)
 
 
 class LeafSurfaces:

@@ -48,7 +48,8 @@ EXPORTS = [
     "stkfmm_p2p_vjp_device", "stkfmm_device_info", "stkfmm_tree_leaf_corners", "stkfmm_surface_count",
     "stkfmm_surface", "stkfmm_box_surfaces_device", "stkfmm_get_s2m_uker", "stkfmm_wall_evaluate",
     "stkfmm_wall_eval_device", "stkfmm_scale_coord", "stkfmm_scale_src", "stkfmm_scale_trg",
-    "stkfmm_scale_coord_device", "stkfmm_scale_src_device", "stkfmm_scale_trg_device",
+    "stkfmm_scale_coord_device", "stkfmm_scale_src_device", "stkfmm_scale_trg_device", "stkfmm_tree_node_info",
+    "stkfmm_tree_get_nodes", "stkfmm_tree_get_wx", "stkfmm_get_s2l_uker", "stkfmm_get_m2t_uker",
 ] + ["stkfmm_ker_" + n for n in KER_T_NAMES]
 
 
@@ -84,6 +85,11 @@ def _declare(L):
     L.stkfmm_tree_get.argtypes = [c_void_p, iptr, iptr, iptr, iptr, iptr, iptr, iptr]
     L.stkfmm_tree_destroy.argtypes = [c_void_p]
     L.stkfmm_tree_leaf_corners.argtypes = [c_void_p, dptr]
+    L.stkfmm_tree_node_info.argtypes = [c_void_p, iptr, iptr]
+    L.stkfmm_tree_get_nodes.argtypes = [c_void_p, dptr, iptr, iptr, iptr, iptr, iptr]
+    L.stkfmm_tree_get_wx.argtypes = [c_void_p, iptr, iptr, iptr, iptr]
+    L.stkfmm_get_s2l_uker.argtypes = [c_uint, c_int, iptr]
+    L.stkfmm_get_m2t_uker.argtypes = [c_uint, iptr]
     L.stkfmm_surface_count.argtypes = [c_int]
     L.stkfmm_surface.argtypes = [c_int, dptr, ctypes.c_double, c_int, dptr]
     L.stkfmm_box_surfaces_device.argtypes = [c_int, ctypes.c_double, c_int, c_void_p, c_void_p, c_void_p, c_void_p]

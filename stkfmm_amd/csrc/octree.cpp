@@ -67,6 +67,8 @@ struct stkfmm_tree {
     std::vector<int> rng_begin, rng_end;
     int max_level = 0;
     long long pairs = 0;
+    // W-list of every leaf (node indices) and X-list of every node (leaf ids), CSR
+    std::vector<int> w_ptr, w_node, x_ptr, x_leaf;
 };
 
 namespace {
@@ -147,6 +149,21 @@ void collect(const stkfmm_tree &t, int node, const Node &b, std::vector<int> &ou
         collect(t, t.children[nd.child_begin + i], b, out);
 }
 
+// W-list of leaf b (the M2T list of the kernel-independent FMM): nodes c deeper than b, not touching
+// b, whose parent touches b -- descendants of b's colleagues that are separated from b while their
+// parents are not.  Walks the nodes that touch b.
+void wlist(const stkfmm_tree &t, int node, const Node &b, std::vector<int> &out) {
+    const Node &nd = t.nodes[node];
+    for (int i = 0; i < nd.n_children; ++i) {
+        const int c = t.children[nd.child_begin + i];
+        const Node &cn = t.nodes[c];
+        if (touches(cn, b))
+            wlist(t, c, b, out);
+        else if (cn.level > b.level)
+            out.push_back(c);
+    }
+}
+
 } // namespace
 
 extern "C" int stkfmm_tree_build(stkfmm_tree **tree_out, int n, const double *coord, int leaf_cap) {
@@ -207,6 +224,25 @@ extern "C" int stkfmm_tree_build(stkfmm_tree **tree_out, int n, const double *co
         pairs += ns * (long long)(b.end - b.begin);
     }
     t->pairs = pairs;
+    // W-lists per leaf; X-lists per node are their transpose (X(c) = leaves whose W-list holds c)
+    const int nn = (int)t->nodes.size();
+    t->w_ptr.assign(1, 0);
+    std::vector<int> xcount(nn + 1, 0);
+    for (int l = 0; l < nl; ++l) {
+        if (!t->nodes.empty())
+            wlist(*t, 0, t->nodes[t->leaf_node[l]], t->w_node);
+        t->w_ptr.push_back((int)t->w_node.size());
+    }
+    for (int c : t->w_node)
+        ++xcount[c + 1];
+    for (int c = 0; c < nn; ++c)
+        xcount[c + 1] += xcount[c];
+    t->x_ptr = xcount;
+    t->x_leaf.assign(t->w_node.size(), 0);
+    std::vector<int> fill(xcount.begin(), xcount.end() - 1);
+    for (int l = 0; l < nl; ++l)
+        for (int k = t->w_ptr[l]; k < t->w_ptr[l + 1]; ++k)
+            t->x_leaf[fill[t->w_node[k]]++] = l;
     *tree_out = t;
     return STKFMM_OK;
 }
@@ -248,6 +284,62 @@ extern "C" int stkfmm_tree_get(const stkfmm_tree *t, int *perm, int *leaf_begin,
         std::copy(t->rng_begin.begin(), t->rng_begin.end(), rng_begin);
     if (rng_end)
         std::copy(t->rng_end.begin(), t->rng_end.end(), rng_end);
+    return STKFMM_OK;
+}
+
+extern "C" int stkfmm_tree_node_info(const stkfmm_tree *t, int *n_nodes, int *n_w) {
+    if (!t)
+        return STKFMM_ERR_ARG;
+    if (n_nodes)
+        *n_nodes = (int)t->nodes.size();
+    if (n_w)
+        *n_w = (int)t->w_node.size();
+    return STKFMM_OK;
+}
+
+extern "C" int stkfmm_tree_get_nodes(const stkfmm_tree *t, double *corner, int *level, int *begin, int *end,
+                                     int *leaf_id, int *parent) {
+    if (!t)
+        return STKFMM_ERR_ARG;
+    const double cell = std::ldexp(1.0, -kBits);
+    const int nn = (int)t->nodes.size();
+    for (int i = 0; i < nn; ++i) {
+        const Node &nd = t->nodes[i];
+        if (corner)
+            for (int k = 0; k < 3; ++k)
+                corner[3 * i + k] = nd.lo[k] * cell;
+        if (level)
+            level[i] = nd.level;
+        if (begin)
+            begin[i] = nd.begin;
+        if (end)
+            end[i] = nd.end;
+        if (leaf_id)
+            leaf_id[i] = nd.leaf_id;
+    }
+    if (parent) {
+        for (int i = 0; i < nn; ++i)
+            parent[i] = -1;
+        for (int i = 0; i < nn; ++i) {
+            const Node &nd = t->nodes[i];
+            for (int c = 0; c < nd.n_children; ++c)
+                parent[t->children[nd.child_begin + c]] = i;
+        }
+    }
+    return STKFMM_OK;
+}
+
+extern "C" int stkfmm_tree_get_wx(const stkfmm_tree *t, int *w_ptr, int *w_node, int *x_ptr, int *x_leaf) {
+    if (!t)
+        return STKFMM_ERR_ARG;
+    if (w_ptr)
+        std::copy(t->w_ptr.begin(), t->w_ptr.end(), w_ptr);
+    if (w_node)
+        std::copy(t->w_node.begin(), t->w_node.end(), w_node);
+    if (x_ptr)
+        std::copy(t->x_ptr.begin(), t->x_ptr.end(), x_ptr);
+    if (x_leaf)
+        std::copy(t->x_leaf.begin(), t->x_leaf.end(), x_leaf);
     return STKFMM_OK;
 }
 

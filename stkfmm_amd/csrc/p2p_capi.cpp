@@ -58,44 +58,51 @@ struct KernelEntry {
     unsigned kernel;
     const char *name; // ker_name given to BuildKernel
     int kdim_sl, kdim_dl, kdim_trg;
-    int uk_sl, uk_dl, uk_l2t; // -1: sub-kernel absent in the reference
-    int uk_s2m_sl, uk_s2m_dl; // k_s2m (source -> upward check surface), single and double layer
+    int uk_sl, uk_dl, uk_l2t; // k_s2t single / double layer, k_l2t; -1: absent in the reference
+    int uk_s2m_sl, uk_s2m_dl; // k_s2m (sources -> upward check surface)
+    int uk_s2l_sl, uk_s2l_dl; // k_s2l (X-list: sources -> downward check surface)
+    int uk_m2t;               // k_m2t (W-list: upward equivalent densities -> targets)
 };
 
 // kdimDL of kernels without a double layer is PVFMM's default surf_dim (PVFMM_COORD_DIM +
 // ker_dim[0]); PVFMM is not vendored, so those entries are inferred, not pinned.
-// Sub-kernels from the BuildKernel lists (a NULL entry means the kernel itself):
-//   LapPGrad / LapPGradGrad: s2m laplace_p + laplace_dipolep (LaplaceLayerKernel.hpp:315-341),
-//     l2t itself;
-//   LapQPGradGrad: s2m laplace_quadp, l2t laplace_pgradgrad (LaplaceLayerKernel.hpp:344-355);
+// Sub-kernels from the BuildKernel lists (argument order s2m, s2l, s2t, m2m, m2l, m2t, l2l, l2t; a
+// NULL entry means the kernel itself):
+//   LapPGrad / LapPGradGrad: s2m = s2l = laplace_p + laplace_dipolep, m2t and l2t itself
+//     (LaplaceLayerKernel.hpp:315-341);
+//   LapQPGradGrad: s2m = s2l = laplace_quadp, m2t = l2t = laplace_pgradgrad (LaplaceLayerKernel.hpp:344-355);
 //   Stokes: everything stokes_vel (StokesLayerKernel.hpp:38-42);
-//   RPY: s2m rpy_u, l2t stk_ulapu (RPYKernel.hpp:136-152);
-//   StokesRegVel: s2m itself, l2t PVFMM's Stokes velocity (StokesRegSingleLayerKernel.hpp:211-218);
-//   StokesRegVelOmega: s2m stokes_regftvel, l2t stokes_velomega (StokesRegSingleLayerKernel.hpp:221-232);
-//   PVel / PVelGrad / PVelLaplacian / Traction: s2m stokes_pvel + stokes_doublepvel, l2t itself
-//     (StokesLayerKernel.hpp:45-88).
+//   RPY: s2m = s2l = rpy_u, m2t = l2t = stk_ulapu (RPYKernel.hpp:136-152);
+//   StokesRegVel: s2m = s2l itself, m2t = l2t PVFMM's Stokes velocity (StokesRegSingleLayerKernel.hpp:211-218);
+//   StokesRegVelOmega: s2m = s2l = stokes_regftvel, m2t = l2t = stokes_velomega
+//     (StokesRegSingleLayerKernel.hpp:221-232);
+//   PVel / PVelGrad / PVelLaplacian / Traction: s2m = s2l = stokes_pvel + stokes_doublepvel, m2t and
+//     l2t itself (StokesLayerKernel.hpp:45-88).
+#define U(x) STKFMM_UKER_##x
 const KernelEntry kTable[] = {
-    {1, "laplace_PGrad", 1, 3, 4, STKFMM_UKER_LAPLACE_PGRAD, STKFMM_UKER_LAPLACE_DIPOLEPGRAD, STKFMM_UKER_LAPLACE_PGRAD,
-     STKFMM_UKER_LAPLACE_P, STKFMM_UKER_LAPLACE_DIPOLEP},
-    {2, "laplace_PGradGrad", 1, 3, 10, STKFMM_UKER_LAPLACE_PGRADGRAD, STKFMM_UKER_LAPLACE_DIPOLEPGRADGRAD,
-     STKFMM_UKER_LAPLACE_PGRADGRAD, STKFMM_UKER_LAPLACE_P, STKFMM_UKER_LAPLACE_DIPOLEP},
-    {4, "laplace_QPGradGrad", 9, 12, 10, STKFMM_UKER_LAPLACE_QUADPGRADGRAD, -1, STKFMM_UKER_LAPLACE_PGRADGRAD,
-     STKFMM_UKER_LAPLACE_QUADP, -1},
-    {8, "stokes_vel", 3, 6, 3, STKFMM_UKER_STOKES_VEL, -1, STKFMM_UKER_STOKES_VEL, STKFMM_UKER_STOKES_VEL, -1},
-    {16, "rpy_ulapu", 4, 7, 6, STKFMM_UKER_RPY_ULAPU, -1, STKFMM_UKER_STK_ULAPU, STKFMM_UKER_RPY_U, -1},
-    {32, "stokes_regvel", 4, 7, 3, STKFMM_UKER_STOKES_REGVEL, -1, STKFMM_UKER_STOKES_VEL, STKFMM_UKER_STOKES_REGVEL,
-     -1},
-    {64, "stokes_regftvelomega", 7, 10, 6, STKFMM_UKER_STOKES_REGFTVELOMEGA, -1, STKFMM_UKER_STOKES_VELOMEGA,
-     STKFMM_UKER_STOKES_REGFTVEL, -1},
-    {128, "stokes_PVel", 4, 9, 4, STKFMM_UKER_STOKES_PVEL, STKFMM_UKER_STOKES_DOUBLEPVEL, STKFMM_UKER_STOKES_PVEL,
-     STKFMM_UKER_STOKES_PVEL, STKFMM_UKER_STOKES_DOUBLEPVEL},
-    {256, "stokes_PVelGrad", 4, 9, 16, STKFMM_UKER_STOKES_PVELGRAD, STKFMM_UKER_STOKES_DOUBLEPVELGRAD,
-     STKFMM_UKER_STOKES_PVELGRAD, STKFMM_UKER_STOKES_PVEL, STKFMM_UKER_STOKES_DOUBLEPVEL},
-    {512, "stokes_PVelLaplacian", 4, 9, 7, STKFMM_UKER_STOKES_PVELLAPLACIAN, STKFMM_UKER_STOKES_DOUBLELAPLACIAN,
-     STKFMM_UKER_STOKES_PVELLAPLACIAN, STKFMM_UKER_STOKES_PVEL, STKFMM_UKER_STOKES_DOUBLEPVEL},
-    {1024, "stokes_Traction", 4, 9, 9, STKFMM_UKER_STOKES_TRACTION, STKFMM_UKER_STOKES_DOUBLETRACTION,
-     STKFMM_UKER_STOKES_TRACTION, STKFMM_UKER_STOKES_PVEL, STKFMM_UKER_STOKES_DOUBLEPVEL},
+    {1, "laplace_PGrad", 1, 3, 4, U(LAPLACE_PGRAD), U(LAPLACE_DIPOLEPGRAD), U(LAPLACE_PGRAD), U(LAPLACE_P),
+     U(LAPLACE_DIPOLEP), U(LAPLACE_P), U(LAPLACE_DIPOLEP), U(LAPLACE_PGRAD)},
+    {2, "laplace_PGradGrad", 1, 3, 10, U(LAPLACE_PGRADGRAD), U(LAPLACE_DIPOLEPGRADGRAD), U(LAPLACE_PGRADGRAD),
+     U(LAPLACE_P), U(LAPLACE_DIPOLEP), U(LAPLACE_P), U(LAPLACE_DIPOLEP), U(LAPLACE_PGRADGRAD)},
+    {4, "laplace_QPGradGrad", 9, 12, 10, U(LAPLACE_QUADPGRADGRAD), -1, U(LAPLACE_PGRADGRAD), U(LAPLACE_QUADP), -1,
+     U(LAPLACE_QUADP), -1, U(LAPLACE_PGRADGRAD)},
+    {8, "stokes_vel", 3, 6, 3, U(STOKES_VEL), -1, U(STOKES_VEL), U(STOKES_VEL), -1, U(STOKES_VEL), -1, U(STOKES_VEL)},
+    {16, "rpy_ulapu", 4, 7, 6, U(RPY_ULAPU), -1, U(STK_ULAPU), U(RPY_U), -1, U(RPY_U), -1, U(STK_ULAPU)},
+    {32, "stokes_regvel", 4, 7, 3, U(STOKES_REGVEL), -1, U(STOKES_VEL), U(STOKES_REGVEL), -1, U(STOKES_REGVEL), -1,
+     U(STOKES_VEL)},
+    {64, "stokes_regftvelomega", 7, 10, 6, U(STOKES_REGFTVELOMEGA), -1, U(STOKES_VELOMEGA), U(STOKES_REGFTVEL), -1,
+     U(STOKES_REGFTVEL), -1, U(STOKES_VELOMEGA)},
+    {128, "stokes_PVel", 4, 9, 4, U(STOKES_PVEL), U(STOKES_DOUBLEPVEL), U(STOKES_PVEL), U(STOKES_PVEL),
+     U(STOKES_DOUBLEPVEL), U(STOKES_PVEL), U(STOKES_DOUBLEPVEL), U(STOKES_PVEL)},
+    {256, "stokes_PVelGrad", 4, 9, 16, U(STOKES_PVELGRAD), U(STOKES_DOUBLEPVELGRAD), U(STOKES_PVELGRAD),
+     U(STOKES_PVEL), U(STOKES_DOUBLEPVEL), U(STOKES_PVEL), U(STOKES_DOUBLEPVEL), U(STOKES_PVELGRAD)},
+    {512, "stokes_PVelLaplacian", 4, 9, 7, U(STOKES_PVELLAPLACIAN), U(STOKES_DOUBLELAPLACIAN),
+     U(STOKES_PVELLAPLACIAN), U(STOKES_PVEL), U(STOKES_DOUBLEPVEL), U(STOKES_PVEL), U(STOKES_DOUBLEPVEL),
+     U(STOKES_PVELLAPLACIAN)},
+    {1024, "stokes_Traction", 4, 9, 9, U(STOKES_TRACTION), U(STOKES_DOUBLETRACTION), U(STOKES_TRACTION),
+     U(STOKES_PVEL), U(STOKES_DOUBLEPVEL), U(STOKES_PVEL), U(STOKES_DOUBLEPVEL), U(STOKES_TRACTION)},
 };
+#undef U
 
 const KernelEntry *find_kernel(unsigned kernel) {
     for (const auto &e : kTable)
@@ -156,6 +163,27 @@ extern "C" int stkfmm_get_s2m_uker(unsigned kernel, int double_layer, int *uker)
         return set_error(STKFMM_ERR_PPKERNEL, "kernel %s has no double-layer S2M sub-kernel", e->name);
     if (uker)
         *uker = uk;
+    return STKFMM_OK;
+}
+
+extern "C" int stkfmm_get_s2l_uker(unsigned kernel, int double_layer, int *uker) {
+    const KernelEntry *e = find_kernel(kernel);
+    if (!e)
+        return set_error(STKFMM_ERR_KERNEL, "Error: Kernel not found.");
+    const int uk = double_layer ? e->uk_s2l_dl : e->uk_s2l_sl;
+    if (uk < 0)
+        return set_error(STKFMM_ERR_PPKERNEL, "kernel %s has no double-layer S2L sub-kernel", e->name);
+    if (uker)
+        *uker = uk;
+    return STKFMM_OK;
+}
+
+extern "C" int stkfmm_get_m2t_uker(unsigned kernel, int *uker) {
+    const KernelEntry *e = find_kernel(kernel);
+    if (!e)
+        return set_error(STKFMM_ERR_KERNEL, "Error: Kernel not found.");
+    if (uker)
+        *uker = e->uk_m2t;
     return STKFMM_OK;
 }
 
