@@ -24,8 +24,9 @@ def problem(kernel, seed, ns=700, nt=500):
         f = np.concatenate([f, rng.uniform(1e-3, 5e-3, (ns, 1))], axis=1)
     t = rng.random((nt, 3))
     t[:, 2] = rng.uniform(0.0, 0.5, nt)
-    t[:8] = x[:8]          # targets at source points
-    t[8:40, 2] = 0.0       # targets on the wall
+    k = min(8, ns, nt)
+    t[:k] = x[:k]          # targets at source points
+    t[k:min(40, nt), 2] = 0.0   # targets on the wall
     return x, f, t
 
 
