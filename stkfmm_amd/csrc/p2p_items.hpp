@@ -70,7 +70,9 @@ __global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemA
     const int lane = threadIdx.x;
     ItemCursor cur(a);
     for (;;) {
-        const int it = next_item(a, cur, lane);
+        // uniform by construction; readfirstlane lets the compiler keep the item in SGPRs (and the
+        // scalar-broadcast variant's record reads as scalar loads)
+        const int it = __builtin_amdgcn_readfirstlane(next_item(a, cur, lane));
         if (it < 0)
             break;
 
