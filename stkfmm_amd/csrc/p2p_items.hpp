@@ -35,15 +35,17 @@ namespace stkp2p {
 #ifndef STKP2P_MIN_WAVES_PER_EU
 #define STKP2P_MIN_WAVES_PER_EU 0
 #endif
-// STKP2P_MIXED_HILO (build option, default 1): in mixed mode each level-scaled coordinate is held
+// STKP2P_MIXED_HILO (build option, default 0): in mixed mode each level-scaled coordinate is held
 // as an FP32 pair hi + lo (hi = fl32(x), lo = fl32(x - hi), ~48 bits together) and the displacement
 // is formed in FP32 as (t_hi - s_hi) + (t_lo - s_lo): the hi difference is exact whenever the two
 // coordinates are within a factor 2 of each other (Sterbenz), the lo difference is tiny, so the
 // result is the FP64 displacement rounded once to FP32 -- as with an FP64 subtraction and a
 // conversion, but in three FP32 operations per axis instead of two FP64-rate ones.  0: FP64
-// coordinates, FP64 subtraction then conversion.
+// coordinates, FP64 subtraction then conversion.  Measured on MI355X (1M PVel U-list, T = 2): the
+// hi/lo form is ~2% slower (more instructions and registers; the FP64 subtraction and conversion
+// cost less than assumed), so it is off by default.
 #ifndef STKP2P_MIXED_HILO
-#define STKP2P_MIXED_HILO 1
+#define STKP2P_MIXED_HILO 0
 #endif
 #if STKP2P_MIN_WAVES_PER_EU > 0
 #define STKP2P_ITEM_ATTR __attribute__((amdgpu_waves_per_eu(STKP2P_MIN_WAVES_PER_EU, 8)))
