@@ -217,11 +217,12 @@ typedef struct stkfmm_tune {
                              several lanes (default); -1: one lane per target always */
     int broadcast;        /* 0: sources staged through LDS per tile (default); 1: per-source records
                              staged in HBM once per execute, read with scalar loads */
-    int queues;           /* persistent mode: work queues, one per XCD by default (8 on MI355X, whose
-                             workgroups are dispatched to the 8 XCDs round-robin); all items of a
-                             target group go to one queue, so the group's sources stay in that XCD's
-                             L2; a wave drains its own queue, then steals from the others.
-                             1: a single cost-sorted queue */
+    int queues;           /* persistent mode: number of work queues (default 1: one cost-sorted queue).
+                             8 on MI355X = one per XCD (workgroups are dispatched to the 8 XCDs
+                             round-robin): all items of a target group go to one queue, so the group's
+                             sources stay in that XCD's L2; a wave drains its own queue, then steals
+                             from the others.  Measured on the 1M PVel U-list: 22% fewer L2 fetches,
+                             1.5% longer (the kernel is VALU-bound) */
 } stkfmm_tune;
 
 typedef struct stkfmm_plan stkfmm_plan;

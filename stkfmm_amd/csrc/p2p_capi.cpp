@@ -448,7 +448,9 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
     // time.  Item slots (and so results) do not depend on the order.
     int n_queues = 1;
     if (persistent == 1) {
-        n_queues = tn.queues > 0 ? tn.queues : ((n_cu % kXcdsPerDevice == 0) ? kXcdsPerDevice : 1);
+        // default: one queue (measured faster on the VALU-bound kernels); tn.queues = kXcdsPerDevice
+        // gives one queue per XCD
+        n_queues = tn.queues > 0 ? tn.queues : 1;
         n_queues = std::max(1, std::min({n_queues, 256, (int)items.size()}));
     }
     std::vector<int> q_ptr(2, 0);
