@@ -2,9 +2,9 @@
 
 For a work list (an octree U-list or a batch of direct sums) and a micro-kernel/precision pair it
 times candidate tunings of ``stkfmm_tune`` -- targets per lane (1/2/4), sources per work item, LDS or
-scalar-broadcast source staging and the scheduling mode -- on the real data, with HIP events on the
-execution stream, and keeps the fastest.  Results are cached in JSON under a key made of the GPU architecture, micro-kernel,
-precision and a coarse signature of the work list (target count, pair count and group count,
+scalar-broadcast source staging, the scheduling mode and one or per-XCD work queues -- on the real
+data, with HIP events on the execution stream, and keeps the fastest.  Results are cached in JSON
+under a key made of the GPU architecture, micro-kernel, precision and a coarse signature of the work list (target count, pair count and group count,
 bucketed by powers of two), so a later plan of the same class reuses the choice without timing.
 
 The cache is read from ``tuned_defaults.json`` next to this file (choices measured on MI355X for

@@ -290,9 +290,6 @@ static void plan_free(stkfmm_plan *p) {
     delete p;
 }
 
-// MI355X: 8 XCDs (each with its own L2); workgroups of a launch are dispatched to them round-robin
-constexpr int kXcdsPerDevice = 8;
-
 template <class T>
 static int upload(T **dst, const std::vector<T> &src, const char *what) {
     *dst = nullptr;
@@ -448,8 +445,8 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
     // time.  Item slots (and so results) do not depend on the order.
     int n_queues = 1;
     if (persistent == 1) {
-        // default: one queue (measured faster on the VALU-bound kernels); tn.queues = kXcdsPerDevice
-        // gives one queue per XCD
+        // default: one queue (measured faster on the VALU-bound kernels); tn.queues = 8 gives one
+        // queue per XCD of MI355X (8 XCDs, each with its own L2, workgroups dealt round-robin)
         n_queues = tn.queues > 0 ? tn.queues : 1;
         n_queues = std::max(1, std::min({n_queues, 256, (int)items.size()}));
     }
