@@ -192,22 +192,33 @@ __global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemA
                                     part[t][k] = 0.0f;
 #pragma unroll 2
                             for (int j = j0 + j_first; j < j1; j += j_step) {
+                                // coordinates first: LDS reads return in issue order, and the
+                                // displacement is needed before the values
+                                float hx, hy, hz, lx, ly, lz;
+                                double sx, sy, sz;
+                                if constexpr (HILO) {
+                                    hx = s_vf[j][0];
+                                    hy = s_vf[j][1];
+                                    hz = s_vf[j][2];
+                                    lx = s_vf[j][3];
+                                    ly = s_vf[j][4];
+                                    lz = s_vf[j][5];
+                                } else {
+                                    sx = s_x[j];
+                                    sy = s_y[j];
+                                    sz = s_z[j];
+                                }
                                 float f[KL];
 #pragma unroll
                                 for (int k = 0; k < KL; ++k)
                                     f[k] = s_vf[j][KOFF + k];
-                                if constexpr (HILO) {
-                                    const float hx = s_vf[j][0], hy = s_vf[j][1], hz = s_vf[j][2];
-                                    const float lx = s_vf[j][3], ly = s_vf[j][4], lz = s_vf[j][5];
 #pragma unroll
-                                    for (int t = 0; t < T; ++t)
+                                for (int t = 0; t < T; ++t) {
+                                    if constexpr (HILO)
                                         K::template eval<float>(part[t], (thx[t] - hx) + (tlx[t] - lx),
                                                                 (thy[t] - hy) + (tly[t] - ly),
                                                                 (thz[t] - hz) + (tlz[t] - lz), f);
-                                } else {
-                                    const double sx = s_x[j], sy = s_y[j], sz = s_z[j];
-#pragma unroll
-                                    for (int t = 0; t < T; ++t)
+                                    else
                                         K::template eval<float>(part[t], (float)(tx[t] - sx), (float)(ty[t] - sy),
                                                                 (float)(tz[t] - sz), f);
                                 }
