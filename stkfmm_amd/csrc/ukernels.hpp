@@ -11,9 +11,10 @@
 //         Lib/src/FMMData.cpp:285-317),
 //   deg(k) homogeneity degree of output k: out(r) = lambda^deg * out(lambda r, scaled sources)
 //         (FMMData::scaleTrg, Lib/src/FMMData.cpp:319-470),
-//   stage(in, out)  per-source precomputation in FP64.  Only sub-expressions the reference
-//         evaluates from source values alone are hoisted here (e.g. sxy + syx, a*a/3), each with
-//         the same operands and operation, so hoisting does not change a single rounding,
+//   stage(in, out)  per-source precomputation in FP64.  Sub-expressions the reference evaluates
+//         from source values alone are hoisted here (e.g. sxy + syx, a*a/3), each with the same
+//         operands and operation, so hoisting does not change a single rounding; the rewritten hot
+//         kernels also stage source-only terms of their rewrite (stated next to their code),
 //   eval<R>(u, dx, dy, dz, f)  u += K(r, f) / post(k) without the scale factor, r = x_trg - x_src,
 //   post(k) constant factor applied to accumulated output k once per work item (so that per-pair
 //         constants such as the 2 of the pressure leave the inner loop).
@@ -306,29 +307,40 @@ struct UkStokesDoublePVel {
 
 // stokes_pvel + stokes_doublepvel at one source point (StokesLayerKernel.hpp:44-50 with SL points =
 // DL points): displacement, r2 and rsqrt are shared by the two layers.  Source values: first array
-// [fx, fy, fz, TrD], second array D row-major (staged as in UkStokesDoublePVel).  With cc = f.r and
-// q = -3 Q/r^5:  p/2 = (cc + tr D)/r^3 + q,   u_i = f_i/r + r_i ((cc - TrD)/r^3 + q);  post()
-// doubles p.
+// [fx, fy, fz, TrD], second array D row-major.  With cc = f.r and Q' = -3 Q (Q = r.D.r):
+//   p/2 = (cc + tr D)/r^3 + Q'/r^5,   u_i = f_i/r + r_i ((cc - TrD)/r^3 + Q'/r^5);  post() doubles p.
+// Rewrite (one factor 1/r^3 pulled out of both layers):
+//   t = (cc - TrD) + Q'/r^2,   u_i = f_i/r + r_i (t/r^3),   p/2 = (t + (TrD + tr D))/r^3,
+// with cc - TrD formed as one fused multiply-add chain started from the staged -TrD, and
+// Q' = dx (g0 dx + g3 dy + g4 dz) + dy (g1 dy + g5 dz) + g2 dz^2 reusing dz^2 of r2.
+// Staged: fx, fy, fz, -TrD, then UkStokesDoublePVel's first six values g0..g5 (the symmetrised,
+// -3-scaled D), then TrD + tr D.  26 operations per pair besides the displacement and the reciprocal
+// square root, against 30 for the two layers' expressions side by side.
 struct UkStokesPVelFused {
     static constexpr int KS = 13, KS1 = 4, KL = 11, KT = 4;
     static constexpr unsigned SRC_SCALE_MASK = 0x1FF8u;
     __host__ __device__ static constexpr int deg(int k) { return k == 0 ? 2 : 1; }
     __host__ __device__ static constexpr double post(int k) { return k == 0 ? 2.0 : 1.0; }
     __device__ static void stage(const double *in, double *o) {
-        o[0] = in[0]; o[1] = in[1]; o[2] = in[2]; o[3] = in[3];
+        o[0] = in[0]; o[1] = in[1]; o[2] = in[2];
+        o[3] = -in[3];
         UkStokesDoublePVel::stage(in + 4, o + 4);
+        o[10] = in[3] + o[10];
     }
     template <class R>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
-        R r2 = dx * dx + dy * dy + dz * dz;
+        R z2 = dz * dz;
+        R r2 = dx * dx + (dy * dy + z2);
         R rinv = rsqrt_masked(r2);
         R rinv2 = rinv * rinv;
         R rinv3 = rinv2 * rinv;
-        R cc = f[0] * dx + f[1] * dy + f[2] * dz;
-        R q = UkStokesDoublePVel::q5(dx, dy, dz, rinv3 * rinv2, f + 4);
-        u[0] += q;
-        u[0] += (cc + f[10]) * rinv3;
-        R b = (cc - f[3]) * rinv3 + q;
+        R cm = f[0] * dx + (f[1] * dy + (f[2] * dz + f[3]));
+        R a1 = f[4] * dx + f[7] * dy + f[8] * dz;
+        R a2 = f[5] * dy + f[9] * dz;
+        R qq = dx * a1 + (dy * a2 + f[6] * z2);
+        R t = qq * rinv2 + cm;
+        R b = t * rinv3;
+        u[0] += (t + f[10]) * rinv3;
         u[1] += f[0] * rinv;
         u[1] += dx * b;
         u[2] += f[1] * rinv;
