@@ -15,14 +15,18 @@
 //         from source values alone are hoisted here (e.g. sxy + syx, a*a/3), each with the same
 //         operands and operation, so hoisting does not change a single rounding; the rewritten hot
 //         kernels also stage source-only terms of their rewrite (stated next to their code),
-//   eval<R>(u, dx, dy, dz, f)  u += K(r, f) / post(k) without the scale factor, r = x_trg - x_src,
+//   eval<R, M>(u, dx, dy, dz, f)  u += K(r, f) / post(k) without the scale factor, r = x_trg -
+//         x_src; M = false for pairs known to have r2 > 0 (no coincidence mask),
 //   post(k) constant factor applied to accumulated output k once per work item (so that per-pair
-//         constants such as the 2 of the pressure leave the inner loop).
+//         constants such as the 2 of the pressure leave the inner loop),
+//   UNMASKED_EVAL (true here: eval<R, false> exists).
 // The hot kernels (Stokeslet, PVel single/double/fused, regularised Stokeslet, RPY, Laplace PGrad)
 // evaluate the reference expression in an equivalent form with fewer operations (common factors
 // pulled out, constants folded into staged source values or post(), each term accumulated by its
-// own fused multiply-add); each such rewrite is stated next to its code.  The remaining
-// kernels follow the reference's algebraic order directly.
+// own fused multiply-add); the Laplace dipole-gradient and gradient-of-gradient kernels are written
+// with no power of 1/r above 3 (or s^4 in unit-direction form), so that FP32 stays finite at close
+// range.  Each rewrite is stated next to its code; the remaining kernels follow the reference's
+// algebraic order directly.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -42,11 +46,15 @@ namespace stkp2p {
 // already 0, so clearing the high word makes the seed +0 and every Newton step keeps it 0 -- one
 // 32-bit select instead of a 64-bit select after the refinement.  (r2 is a sum of squares: +0 or
 // positive, never -0 or negative.)
-__device__ __forceinline__ double rsqrt_masked(double r2) {
+// MASK = false drops the select, for pairs known to have r2 > 0 (the result is then the same).
+template <bool MASK = true>
+__device__ __forceinline__ double rsqrt_f64(double r2) {
     double y = __builtin_amdgcn_rsq(r2);
-    unsigned long long bits = __builtin_bit_cast(unsigned long long, y);
-    const unsigned hi = (r2 > 0.0) ? (unsigned)(bits >> 32) : 0u;
-    y = __builtin_bit_cast(double, ((unsigned long long)hi << 32) | (bits & 0xffffffffull));
+    if constexpr (MASK) {
+        unsigned long long bits = __builtin_bit_cast(unsigned long long, y);
+        const unsigned hi = (r2 > 0.0) ? (unsigned)(bits >> 32) : 0u;
+        y = __builtin_bit_cast(double, ((unsigned long long)hi << 32) | (bits & 0xffffffffull));
+    }
     const double hr = 0.5 * r2;
 #pragma unroll
     for (int i = 0; i < STKP2P_FP64_NEWTON; ++i) {
@@ -56,14 +64,48 @@ __device__ __forceinline__ double rsqrt_masked(double r2) {
     return y;
 }
 
-// FP32: v_rsq_f32 seed (measured: 23.2 bits) and one Newton step.
-__device__ __forceinline__ float rsqrt_masked(float r2) {
+// FP32: v_rsq_f32 seed (measured on gfx950: 23.2 correct bits, i.e. within 1.1e-7 relative) and
+// one Newton step.
+template <bool MASK = true>
+__device__ __forceinline__ float rsqrt_f32(float r2) {
     float y = __builtin_amdgcn_rsqf(r2);
     const float hr = 0.5f * r2;
     const float e = __builtin_fmaf(-hr, y * y, 0.5f);
     y = __builtin_fmaf(y, e, y);
-    return (r2 > 0.0f) ? y : 0.0f;
+    if constexpr (MASK)
+        return (r2 > 0.0f) ? y : 0.0f;
+    else
+        return y;
 }
+
+// masked and refined (the adjoint kernels and the rest of the family)
+__device__ __forceinline__ double rsqrt_masked(double r2) { return rsqrt_f64<true>(r2); }
+__device__ __forceinline__ float rsqrt_masked(float r2) { return rsqrt_f32<true>(r2); }
+
+// 1/r and 1/r^2 of a pair, for the micro-kernels with the eval<R, M> interface: masked unless the
+// caller has shown r2 > 0 (M = false); s Newton-refined (FP32 and FP64), s2 = s s.
+// (Measured on MI355X: the FP32 seed alone, with s2 corrected to FP32 rounding, is 2 VALU operations
+// cheaper per pair and leaves small sums at their accuracy, but its error is not zero-mean: summed
+// over the ~14,500 sources per target of the 4M-point RPY U-list it raised errorL2 from 4.2e-7 to
+// 1.3e-6.  The Newton step stays.)
+template <class R> struct RsqPair { R s, s2; };
+template <bool M>
+__device__ __forceinline__ RsqPair<double> pair_rsqrt(double r2) {
+    const double s = rsqrt_f64<M>(r2);
+    return {s, s * s};
+}
+template <bool M>
+__device__ __forceinline__ RsqPair<float> pair_rsqrt(float r2) {
+    const float s = rsqrt_f32<M>(r2);
+    return {s, s * s};
+}
+
+// Micro-kernels that declare UNMASKED_EVAL provide eval<R, M> with M = false for tiles in which
+// every pair has r2 > 0 (the item kernel checks this per source tile); the others provide eval<R>.
+template <class K, class = void> struct HasUnmaskedEval { static constexpr bool value = false; };
+template <class K> struct HasUnmaskedEval<K, decltype((void)K::UNMASKED_EVAL)> {
+    static constexpr bool value = K::UNMASKED_EVAL;
+};
 
 // reference constant one_over_three = 0.3333333333333 (RPYKernel.hpp:30,66)
 #define STKP2P_RPY_ONE_OVER_THREE 0.3333333333333
@@ -73,14 +115,15 @@ __device__ __forceinline__ float rsqrt_masked(float r2) {
 // laplace_p, LaplaceLayerKernel.hpp:58-64
 struct UkLaplaceP {
     static constexpr int KS = 1, KS1 = 1, KL = 1, KT = 1;
+    static constexpr bool UNMASKED_EVAL = true;
     static constexpr unsigned SRC_SCALE_MASK = 0u;
     __host__ __device__ static constexpr int deg(int) { return 1; }
     __host__ __device__ static constexpr double post(int k) { return ((void)k, 1.0); }
     __device__ static void stage(const double *in, double *o) { o[0] = in[0]; }
-    template <class R>
+    template <class R, bool M = true>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
-        R rinv = rsqrt_masked(r2);
+        R rinv = pair_rsqrt<M>(r2).s;
         u[0] += f[0] * rinv;
     }
 };
@@ -88,15 +131,16 @@ struct UkLaplaceP {
 // laplace_dipolep, LaplaceLayerKernel.hpp:104-112
 struct UkLaplaceDipoleP {
     static constexpr int KS = 3, KS1 = 3, KL = 3, KT = 1;
+    static constexpr bool UNMASKED_EVAL = true;
     static constexpr unsigned SRC_SCALE_MASK = 0x7u;
     __host__ __device__ static constexpr int deg(int) { return 1; }
     __host__ __device__ static constexpr double post(int k) { return ((void)k, 1.0); }
     __device__ static void stage(const double *in, double *o) { o[0] = in[0]; o[1] = in[1]; o[2] = in[2]; }
-    template <class R>
+    template <class R, bool M = true>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
-        R rinv = rsqrt_masked(r2);
-        R rinv3 = rinv * rinv * rinv;
+        const auto rs = pair_rsqrt<M>(r2);
+        R rinv3 = rs.s2 * rs.s;
         R rdotn = f[0] * dx + f[1] * dy + f[2] * dz;
         u[0] += rinv3 * rdotn;
     }
@@ -107,16 +151,17 @@ struct UkLaplaceDipoleP {
 // positive sign and negated by post().
 struct UkLaplacePGrad {
     static constexpr int KS = 1, KS1 = 1, KL = 1, KT = 4;
+    static constexpr bool UNMASKED_EVAL = true;
     static constexpr unsigned SRC_SCALE_MASK = 0u;
     __host__ __device__ static constexpr int deg(int k) { return k == 0 ? 1 : 2; }
     __host__ __device__ static constexpr double post(int k) { return k == 0 ? 1.0 : -1.0; }
     __device__ static void stage(const double *in, double *o) { o[0] = in[0]; }
-    template <class R>
+    template <class R, bool M = true>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
-        R rinv = rsqrt_masked(r2);
-        R qr = f[0] * rinv;
-        R sv = qr * (rinv * rinv);
+        const auto rs = pair_rsqrt<M>(r2);
+        R qr = f[0] * rs.s;
+        R sv = qr * rs.s2;
         u[0] += qr;
         u[1] += sv * dx;
         u[2] += sv * dy;
@@ -124,88 +169,104 @@ struct UkLaplacePGrad {
     }
 };
 
-// laplace_dipolepgrad, LaplaceLayerKernel.hpp:121-135
+// laplace_dipolepgrad, LaplaceLayerKernel.hpp:121-135; dipole n:
+//   p = (r.n)/r^3,  grad p_i = (n_i r^2 - 3 (r.n) r_i)/r^5.
+// Rewrite: w = (r.n)/r^2; p = w/r, grad p_i = (n_i - 3 w r_i)/r^3 -- no power above 1/r^3, so the
+// FP32 form stays finite down to r ~ 1e-12 (1/r^5 overflows FP32 below r ~ 2e-8).
 struct UkLaplaceDipolePGrad {
     static constexpr int KS = 3, KS1 = 3, KL = 3, KT = 4;
+    static constexpr bool UNMASKED_EVAL = true;
     static constexpr unsigned SRC_SCALE_MASK = 0x7u;
     __host__ __device__ static constexpr int deg(int k) { return k == 0 ? 1 : 2; }
     __host__ __device__ static constexpr double post(int k) { return ((void)k, 1.0); }
     __device__ static void stage(const double *in, double *o) { o[0] = in[0]; o[1] = in[1]; o[2] = in[2]; }
-    template <class R>
+    template <class R, bool M = true>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
-        R rinv = rsqrt_masked(r2);
-        R rinv2 = rinv * rinv;
+        const auto rs = pair_rsqrt<M>(r2);
+        R rinv = rs.s, rinv2 = rs.s2;
         R rinv3 = rinv2 * rinv;
-        R rinv5 = rinv3 * rinv2;
-        R rdotn = f[0] * dx + f[1] * dy + f[2] * dz;
-        const R three = R(3.0);
-        u[0] += rdotn * rinv3;
-        u[1] += (f[0] * r2 - three * rdotn * dx) * rinv5;
-        u[2] += (f[1] * r2 - three * rdotn * dy) * rinv5;
-        u[3] += (f[2] * r2 - three * rdotn * dz) * rinv5;
+        R w = (f[0] * dx + f[1] * dy + f[2] * dz) * rinv2;
+        R m = R(-3.0) * w;
+        u[0] += w * rinv;
+        u[1] += (m * dx + f[0]) * rinv3;
+        u[2] += (m * dy + f[1]) * rinv3;
+        u[3] += (m * dz + f[2]) * rinv3;
     }
 };
 
-// laplace_pgradgrad, LaplaceLayerKernel.hpp:144-165
+// laplace_pgradgrad, LaplaceLayerKernel.hpp:144-165; charge q:
+//   p = q/r,  grad p = -q r/r^3,  grad grad p_ij = q (3 r_i r_j - r^2 delta_ij)/r^5.
+// Unit-direction form (s = 1/r, n = r s): grad p = -(q s^3) r,  grad grad p_ij = q s^3 (3 n_i n_j -
+// delta_ij) -- no power above s^3 (the literal 1/r^5 overflows FP32 below r ~ 2e-8).
 struct UkLaplacePGradGrad {
     static constexpr int KS = 1, KS1 = 1, KL = 1, KT = 10;
+    static constexpr bool UNMASKED_EVAL = true;
     static constexpr unsigned SRC_SCALE_MASK = 0u;
     __host__ __device__ static constexpr int deg(int k) { return k == 0 ? 1 : (k < 4 ? 2 : 3); }
     __host__ __device__ static constexpr double post(int k) { return ((void)k, 1.0); }
     __device__ static void stage(const double *in, double *o) { o[0] = in[0]; }
-    template <class R>
+    template <class R, bool M = true>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
-        R rinv = rsqrt_masked(r2);
-        R rinv2 = rinv * rinv;
-        R rinv3 = rinv2 * rinv;
-        const R three = R(3.0);
-        R sv = f[0] * rinv3;
-        u[0] += sv * r2;
-        u[1] -= sv * dx;
-        u[2] -= sv * dy;
-        u[3] -= sv * dz;
-        sv *= rinv2;
-        u[4] += sv * (three * dx * dx - r2);
-        u[5] += sv * three * dx * dy;
-        u[6] += sv * three * dx * dz;
-        u[7] += sv * (three * dy * dy - r2);
-        u[8] += sv * three * dy * dz;
-        u[9] += sv * (three * dz * dz - r2);
+        const auto rs = pair_rsqrt<M>(r2);
+        R s = rs.s;
+        R qs = f[0] * s;
+        R qs3 = qs * rs.s2;
+        u[0] += qs;
+        u[1] -= qs3 * dx;
+        u[2] -= qs3 * dy;
+        u[3] -= qs3 * dz;
+        const R nx = dx * s, ny = dy * s, nz = dz * s;
+        const R t3 = R(3.0) * qs3;
+        const R ax = t3 * nx, ay = t3 * ny, az = t3 * nz;
+        u[4] += ax * nx - qs3;
+        u[5] += ax * ny;
+        u[6] += ax * nz;
+        u[7] += ay * ny - qs3;
+        u[8] += ay * nz;
+        u[9] += az * nz - qs3;
     }
 };
 
-// laplace_dipolepgradgrad, LaplaceLayerKernel.hpp:174-199
+// laplace_dipolepgradgrad, LaplaceLayerKernel.hpp:174-199; dipole f, c = n.f (s = 1/r, n = r s):
+//   p = c s^2,  grad p_i = (f_i - 3 c n_i) s^3,
+//   grad grad p_ij = (15 c n_i n_j - 3 c delta_ij - 3 (n_i f_j + n_j f_i)) s^4
+// (the reference's r-polynomials over r^3, r^5, r^7 divided through by r^k; the literal 1/r^7
+// overflows FP32 below r ~ 3e-6, the unit-direction form only below r ~ 1e-9).
 struct UkLaplaceDipolePGradGrad {
     static constexpr int KS = 3, KS1 = 3, KL = 3, KT = 10;
+    static constexpr bool UNMASKED_EVAL = true;
     static constexpr unsigned SRC_SCALE_MASK = 0x7u;
     __host__ __device__ static constexpr int deg(int k) { return k == 0 ? 1 : (k < 4 ? 2 : 3); }
     __host__ __device__ static constexpr double post(int k) { return ((void)k, 1.0); }
     __device__ static void stage(const double *in, double *o) { o[0] = in[0]; o[1] = in[1]; o[2] = in[2]; }
-    template <class R>
+    template <class R, bool M = true>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
-        const R three = R(3.0);
-        const R threer2 = three * r2;
-        const R six = R(6.0);
-        const R fifteen = R(15.0);
-        R rinv = rsqrt_masked(r2);
-        R rinv2 = rinv * rinv;
-        R rinv3 = rinv2 * rinv;
-        R rinv5 = rinv2 * rinv3;
-        R rinv7 = rinv2 * rinv5;
-        R rdotn = dx * f[0] + dy * f[1] + dz * f[2];
-        u[0] += rdotn * rinv3;
-        u[1] += (f[0] * r2 - rdotn * three * dx) * rinv5;
-        u[2] += (f[1] * r2 - rdotn * three * dy) * rinv5;
-        u[3] += (f[2] * r2 - rdotn * three * dz) * rinv5;
-        u[4] += (fifteen * dx * dx * rdotn - r2 * (three * rdotn + six * dx * f[0])) * rinv7;
-        u[5] += (fifteen * dx * dy * rdotn - threer2 * (dx * f[1] + dy * f[0])) * rinv7;
-        u[6] += (fifteen * dx * dz * rdotn - threer2 * (dx * f[2] + dz * f[0])) * rinv7;
-        u[7] += (fifteen * dy * dy * rdotn - r2 * (three * rdotn + six * dy * f[1])) * rinv7;
-        u[8] += (fifteen * dy * dz * rdotn - threer2 * (dy * f[2] + dz * f[1])) * rinv7;
-        u[9] += (fifteen * dz * dz * rdotn - r2 * (three * rdotn + six * dz * f[2])) * rinv7;
+        const auto rs = pair_rsqrt<M>(r2);
+        const R s = rs.s, s2 = rs.s2;
+        const R n[3] = {dx * s, dy * s, dz * s};
+        const R c = f[0] * n[0] + f[1] * n[1] + f[2] * n[2];
+        const R s3 = s2 * s;
+        const R s4 = s2 * s2;
+        const R m3 = R(-3.0) * c;
+        const R c15 = R(15.0) * c;
+        u[0] += c * s2;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            u[1 + i] += (m3 * n[i] + f[i]) * s3;
+        R e[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            e[i] = c15 * n[i];
+        // output order xx, xy, xz, yy, yz, zz
+        u[4] += ((e[0] * n[0] + m3) - R(6.0) * n[0] * f[0]) * s4;
+        u[5] += (e[0] * n[1] - R(3.0) * (n[0] * f[1] + n[1] * f[0])) * s4;
+        u[6] += (e[0] * n[2] - R(3.0) * (n[0] * f[2] + n[2] * f[0])) * s4;
+        u[7] += ((e[1] * n[1] + m3) - R(6.0) * n[1] * f[1]) * s4;
+        u[8] += (e[1] * n[2] - R(3.0) * (n[1] * f[2] + n[2] * f[1])) * s4;
+        u[9] += ((e[2] * n[2] + m3) - R(6.0) * n[2] * f[2]) * s4;
     }
 };
 
@@ -215,15 +276,17 @@ struct UkLaplaceDipolePGradGrad {
 // Test/SimpleKernel.cpp:986-1023 with a = 0): u = f/r + r (f.r)/r^3.
 struct UkStokesVel {
     static constexpr int KS = 3, KS1 = 3, KL = 3, KT = 3;
+    static constexpr bool UNMASKED_EVAL = true;
     static constexpr unsigned SRC_SCALE_MASK = 0u;
     __host__ __device__ static constexpr int deg(int) { return 1; }
     __host__ __device__ static constexpr double post(int k) { return ((void)k, 1.0); }
     __device__ static void stage(const double *in, double *o) { o[0] = in[0]; o[1] = in[1]; o[2] = in[2]; }
-    template <class R>
+    template <class R, bool M = true>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
-        R rinv = rsqrt_masked(r2);
-        R rinv3 = rinv * rinv * rinv;
+        const auto rs = pair_rsqrt<M>(r2);
+        R rinv = rs.s;
+        R rinv3 = rs.s2 * rinv;
         R fdotr = f[0] * dx + f[1] * dy + f[2] * dz;
         R c = fdotr * rinv3;
         u[0] += f[0] * rinv;
@@ -240,17 +303,19 @@ struct UkStokesVel {
 // Rewrite: r^2/r^3 = 1/r; the factor 2 of p is applied by post().
 struct UkStokesPVel {
     static constexpr int KS = 4, KS1 = 4, KL = 4, KT = 4;
+    static constexpr bool UNMASKED_EVAL = true;
     static constexpr unsigned SRC_SCALE_MASK = 0x8u;
     __host__ __device__ static constexpr int deg(int k) { return k == 0 ? 2 : 1; }
     __host__ __device__ static constexpr double post(int k) { return k == 0 ? 2.0 : 1.0; }
     __device__ static void stage(const double *in, double *o) {
         o[0] = in[0]; o[1] = in[1]; o[2] = in[2]; o[3] = in[3];
     }
-    template <class R>
+    template <class R, bool M = true>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
-        R rinv = rsqrt_masked(r2);
-        R rinv3 = rinv * (rinv * rinv);
+        const auto rs = pair_rsqrt<M>(r2);
+        R rinv = rs.s;
+        R rinv3 = rs.s2 * rinv;
         R cc = f[0] * dx + f[1] * dy + f[2] * dz;
         u[0] += cc * rinv3;
         R b = (cc - f[3]) * rinv3;
@@ -267,9 +332,11 @@ struct UkStokesPVel {
 //   Q = r.D.r,  p = 2 (tr D/r^3 - 3 Q/r^5),  u_i = -3 r_i Q/r^5.
 // Staged: -3 Dxx, -3 Dyy, -3 Dzz, -3 (Dxy+Dyx), -3 (Dxz+Dzx), -3 (Dyz+Dzy), Dxx+Dyy+Dzz (the
 // reference's source-only sub-expressions, :46-48,51, with its factor -3 of :49 folded in).
-// Rewrite: q = -3 Q/r^5 by Horner in dx, dy, dz; p/2 = tr D/r^3 + q, u_i = r_i q; post() doubles p.
+// Rewrite: Q' = -3 Q by Horner in dx, dy, dz (reusing dz^2 of r2), t = Q'/r^2, q = t/r^3 (= -3 Q/r^5
+// with no power above 1/r^3), p/2 = (t + tr D)/r^3, u_i = r_i q; post() doubles p.
 struct UkStokesDoublePVel {
     static constexpr int KS = 9, KS1 = 9, KL = 7, KT = 4;
+    static constexpr bool UNMASKED_EVAL = true;
     static constexpr unsigned SRC_SCALE_MASK = 0x1FFu;
     __host__ __device__ static constexpr int deg(int k) { return k == 0 ? 2 : 1; }
     __host__ __device__ static constexpr double post(int k) { return k == 0 ? 2.0 : 1.0; }
@@ -282,23 +349,19 @@ struct UkStokesDoublePVel {
         o[5] = -3.0 * (in[5] + in[7]);
         o[6] = in[0] + in[4] + in[8];
     }
-    // -3 Q/r^5 from staged values
-    template <class R>
-    __device__ static R q5(R dx, R dy, R dz, R rinv5, const R *g) {
+    template <class R, bool M = true>
+    __device__ static void eval(R *u, R dx, R dy, R dz, const R *g) {
+        R z2 = dz * dz;
+        R r2 = dx * dx + (dy * dy + z2);
+        const auto rs = pair_rsqrt<M>(r2);
+        R rinv = rs.s, rinv2 = rs.s2;
+        R rinv3 = rinv2 * rinv;
         R a1 = g[0] * dx + g[3] * dy + g[4] * dz;
         R a2 = g[1] * dy + g[5] * dz;
-        R a3 = g[2] * dz;
-        return (dx * a1 + dy * a2 + dz * a3) * rinv5;
-    }
-    template <class R>
-    __device__ static void eval(R *u, R dx, R dy, R dz, const R *g) {
-        R r2 = dx * dx + dy * dy + dz * dz;
-        R rinv = rsqrt_masked(r2);
-        R rinv2 = rinv * rinv;
-        R rinv3 = rinv2 * rinv;
-        R q = q5(dx, dy, dz, rinv3 * rinv2, g);
-        u[0] += q;
-        u[0] += g[6] * rinv3;
+        R qq = dx * a1 + (dy * a2 + g[2] * z2);
+        R t = qq * rinv2;
+        R q = t * rinv3;
+        u[0] += (t + g[6]) * rinv3;
         u[1] += dx * q;
         u[2] += dy * q;
         u[3] += dz * q;
@@ -318,6 +381,7 @@ struct UkStokesDoublePVel {
 // square root, against 30 for the two layers' expressions side by side.
 struct UkStokesPVelFused {
     static constexpr int KS = 13, KS1 = 4, KL = 11, KT = 4;
+    static constexpr bool UNMASKED_EVAL = true;
     static constexpr unsigned SRC_SCALE_MASK = 0x1FF8u;
     __host__ __device__ static constexpr int deg(int k) { return k == 0 ? 2 : 1; }
     __host__ __device__ static constexpr double post(int k) { return k == 0 ? 2.0 : 1.0; }
@@ -327,12 +391,12 @@ struct UkStokesPVelFused {
         UkStokesDoublePVel::stage(in + 4, o + 4);
         o[10] = in[3] + o[10];
     }
-    template <class R>
+    template <class R, bool M = true>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R z2 = dz * dz;
         R r2 = dx * dx + (dy * dy + z2);
-        R rinv = rsqrt_masked(r2);
-        R rinv2 = rinv * rinv;
+        const auto rs = pair_rsqrt<M>(r2);
+        R rinv = rs.s, rinv2 = rs.s2;
         R rinv3 = rinv2 * rinv;
         R cm = f[0] * dx + (f[1] * dy + (f[2] * dz + f[3]));
         R a1 = f[4] * dx + f[7] * dy + f[8] * dz;
@@ -355,18 +419,19 @@ struct UkStokesPVelFused {
 // Staged: fx, fy, fz, eps*eps.  The mask is on the regularised rho2.
 struct UkStokesRegVel {
     static constexpr int KS = 4, KS1 = 4, KL = 4, KT = 3;
+    static constexpr bool UNMASKED_EVAL = true;
     static constexpr unsigned SRC_SCALE_MASK = 0x8u;
     __host__ __device__ static constexpr int deg(int) { return 1; }
     __host__ __device__ static constexpr double post(int k) { return ((void)k, 1.0); }
     __device__ static void stage(const double *in, double *o) {
         o[0] = in[0]; o[1] = in[1]; o[2] = in[2]; o[3] = in[3] * in[3];
     }
-    template <class R>
+    template <class R, bool M = true>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         const R reg2 = f[3];
         R r2 = dx * dx + dy * dy + dz * dz + reg2;
-        R rinv = rsqrt_masked(r2);
-        R rinv3 = rinv * rinv * rinv;
+        const auto rs = pair_rsqrt<M>(r2);
+        R rinv3 = rs.s2 * rs.s;
         R a = (r2 + reg2) * rinv3;
         R b = (f[0] * dx + f[1] * dy + f[2] * dz) * rinv3;
         u[0] += f[0] * a;
@@ -384,6 +449,7 @@ struct UkStokesRegVel {
 // Staged: fx, fy, fz, beta, -3 beta.
 struct UkRpyU {
     static constexpr int KS = 4, KS1 = 4, KL = 5, KT = 3;
+    static constexpr bool UNMASKED_EVAL = true;
     static constexpr unsigned SRC_SCALE_MASK = 0x8u;
     __host__ __device__ static constexpr int deg(int) { return 1; }
     __host__ __device__ static constexpr double post(int k) { return ((void)k, 1.0); }
@@ -392,11 +458,11 @@ struct UkRpyU {
         o[3] = STKP2P_RPY_ONE_OVER_THREE * in[3] * in[3];
         o[4] = -3.0 * o[3];
     }
-    template <class R>
+    template <class R, bool M = true>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
-        R rinv = rsqrt_masked(r2);
-        R rinv2 = rinv * rinv;
+        const auto rs = pair_rsqrt<M>(r2);
+        R rinv = rs.s, rinv2 = rs.s2;
         R rinv3 = rinv2 * rinv;
         R fr = f[0] * dx + f[1] * dy + f[2] * dz;
         R g3 = fr * rinv3;
@@ -417,15 +483,16 @@ struct UkRpyU {
 // (g3 = (f.r)/r^3, g5 = (f.r)/r^5); post() doubles lap u.  Staged as UkRpyU.
 struct UkRpyULapU {
     static constexpr int KS = 4, KS1 = 4, KL = 5, KT = 6;
+    static constexpr bool UNMASKED_EVAL = true;
     static constexpr unsigned SRC_SCALE_MASK = 0x8u;
     __host__ __device__ static constexpr int deg(int k) { return k < 3 ? 1 : 3; }
     __host__ __device__ static constexpr double post(int k) { return k < 3 ? 1.0 : 2.0; }
     __device__ static void stage(const double *in, double *o) { UkRpyU::stage(in, o); }
-    template <class R>
+    template <class R, bool M = true>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
-        R rinv = rsqrt_masked(r2);
-        R rinv2 = rinv * rinv;
+        const auto rs = pair_rsqrt<M>(r2);
+        R rinv = rs.s, rinv2 = rs.s2;
         R rinv3 = rinv2 * rinv;
         R fr = f[0] * dx + f[1] * dy + f[2] * dz;
         R g3 = fr * rinv3;
@@ -451,15 +518,16 @@ struct UkRpyULapU {
 // stk_ulapu, RPYKernel.hpp:103-128; source [fx, fy, fz]: rpy_ulapu with beta = 0.
 struct UkStkULapU {
     static constexpr int KS = 3, KS1 = 3, KL = 3, KT = 6;
+    static constexpr bool UNMASKED_EVAL = true;
     static constexpr unsigned SRC_SCALE_MASK = 0u;
     __host__ __device__ static constexpr int deg(int k) { return k < 3 ? 1 : 3; }
     __host__ __device__ static constexpr double post(int k) { return k < 3 ? 1.0 : 2.0; }
     __device__ static void stage(const double *in, double *o) { o[0] = in[0]; o[1] = in[1]; o[2] = in[2]; }
-    template <class R>
+    template <class R, bool M = true>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
-        R rinv = rsqrt_masked(r2);
-        R rinv2 = rinv * rinv;
+        const auto rs = pair_rsqrt<M>(r2);
+        R rinv = rs.s, rinv2 = rs.s2;
         R rinv3 = rinv2 * rinv;
         R fr = f[0] * dx + f[1] * dy + f[2] * dz;
         R g3 = fr * rinv3;

@@ -7,7 +7,7 @@
 using namespace stkp2p;
 
 namespace {
-template <class K, class R>
+template <class K, class R, bool MASK = true>
 void pair(const double *r, const double *fin, double scale, double *u) {
     double vst[K::KL];
     double vin[K::KS];
@@ -20,23 +20,32 @@ void pair(const double *r, const double *fin, double scale, double *u) {
     R acc[K::KT];
     for (int k = 0; k < K::KT; ++k)
         acc[k] = R(0);
-    K::template eval<R>(acc, (R)r[0], (R)r[1], (R)r[2], f);
+    if constexpr (HasUnmaskedEval<K>::value)
+        K::template eval<R, MASK>(acc, (R)r[0], (R)r[1], (R)r[2], f);
+    else
+        K::template eval<R>(acc, (R)r[0], (R)r[1], (R)r[2], f);
     for (int k = 0; k < K::KT; ++k)
         u[k] = (double)acc[k] * K::post(k) * scale;
 }
 
 template <class K>
 int run(int prec, const double *r, const double *f, double *u) {
-    if (prec == 0)
-        pair<K, double>(r, f, 1.0, u);
-    else
-        pair<K, float>(r, f, 1.0, u);
+    // prec bit 0: FP32; bit 1: the unmasked form (micro-kernels that offer it; -2 otherwise)
+    if ((prec & 2) && !HasUnmaskedEval<K>::value)
+        return -2;
+    switch (prec) {
+    case 0: pair<K, double>(r, f, 1.0, u); break;
+    case 1: pair<K, float>(r, f, 1.0, u); break;
+    case 2: pair<K, double, false>(r, f, 1.0, u); break;
+    default: pair<K, float, false>(r, f, 1.0, u); break;
+    }
     return K::KT;
 }
 } // namespace
 
-// u = the micro-kernel's sum for one pair (no ScaleFactor), prec 0 = FP64, 1 = FP32 arithmetic;
-// returns the number of outputs or -1 for an unknown id
+// u = the micro-kernel's sum for one pair (no ScaleFactor), prec 0 = FP64, 1 = FP32 arithmetic,
+// 2 / 3 = the same without the coincidence mask (eval<R, false>, for pairs with r2 > 0); returns the
+// number of outputs, -1 for an unknown id, -2 when the micro-kernel has no unmasked form
 extern "C" int ukernel_host_pair(int uk, int prec, const double *r, const double *f, double *u) {
     switch (uk) {
     case 0: return run<UkLaplaceP>(prec, r, f, u);

@@ -28,13 +28,18 @@ def lib():
     return L
 
 
-def host_pair(uk, prec, r, f):
+def host_pair_raw(uk, prec, r, f):
     r = np.ascontiguousarray(r, dtype=np.float64)
     f = np.ascontiguousarray(f, dtype=np.float64)
     u = np.zeros(32)
     kt = lib().ukernel_host_pair(uk, prec, r.ctypes.data_as(_dp), f.ctypes.data_as(_dp), u.ctypes.data_as(_dp))
+    return kt, u[:max(kt, 0)]
+
+
+def host_pair(uk, prec, r, f):
+    kt, u = host_pair_raw(uk, prec, r, f)
     assert kt == O.trg_dim(uk)
-    return u[:kt]
+    return u
 
 
 def values(uk, rng):
@@ -89,3 +94,39 @@ def test_fp64_close_pairs_match_reference_rounding(uk):
         f = values(uk, rng)
         a, b = host_pair(uk, 0, r, f), O.pair(uk, r, f)
         np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-12 * np.abs(b).max())
+
+
+@pytest.mark.parametrize("uk", range(25))
+def test_unmasked_form_equals_masked_for_separated_pairs(uk):
+    # the item kernel runs eval<R, false> on source tiles shown to hold no coincident pair; for
+    # r2 > 0 it must give the masked form's result bit for bit, in FP64 and FP32
+    rng = np.random.default_rng(700 + uk)
+    r0 = np.full(3, 0.5)
+    if host_pair_raw(uk, 2, r0, values(uk, rng))[0] == -2:
+        pytest.skip("micro-kernel without an unmasked form")
+    for _ in range(100):
+        r = rng.uniform(-1, 1, 3) * 10 ** rng.uniform(-6, 1)
+        f = values(uk, rng)
+        for prec in (0, 1):
+            kt, a = host_pair_raw(uk, prec, r, f)
+            kt2, b = host_pair_raw(uk, prec | 2, r, f)
+            assert kt == kt2 and np.array_equal(a, b), (prec, r)
+
+
+CORE = range(14)
+
+
+@pytest.mark.parametrize("uk", CORE)
+def test_fp32_close_range_finite(uk):
+    # level-scaled pair distances down to 1e-8: no power of 1/r above what FP32 holds there (the
+    # literal r^-5 / r^-7 of the Laplace dipole-gradient and gradient-of-gradient kernels overflowed)
+    rng = np.random.default_rng(800 + uk)
+    for _ in range(200):
+        r = rng.uniform(-1, 1, 3) * 10 ** rng.uniform(-8, -5)
+        f = values(uk, rng)
+        a, b = host_pair(uk, 1, r, f), O.pair(uk, r, f)
+        assert np.all(np.isfinite(a)), r
+        # FP32 rounding amplified by cancellation (the double-layer pressure tr D - 3 n.D.n can be
+        # 1e-3 of its terms); a literal power that overflowed or lost range would miss by far more
+        scale = np.abs(b).max()
+        assert np.abs(a - b).max() <= 2e-4 * scale, (r, np.abs(a - b).max() / scale)
