@@ -14,6 +14,8 @@ constexpr int kSrcTile = 64;
 // mixed mode: FP32 partial sums over this many sources before they are added to the FP64
 // accumulator of the target
 constexpr int kMixedAccBlock = 16;
+static_assert(kSrcTile % kMixedAccBlock == 0 && kWave % kMixedAccBlock == 0,
+              "source tiles and chunks hold whole partial-sum blocks");
 
 // one work item: a target tile against a list of source ranges
 struct Item {

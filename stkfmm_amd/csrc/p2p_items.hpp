@@ -13,12 +13,9 @@
 // in FP64, micro-kernel (rsqrt + 1 Newton step, polynomial) in FP32, FP32 partial sums over
 // kMixedAccBlock sources added into FP64 per-target accumulators.
 //
-// Coincidence mask: the reference zeroes pairs with r2 = 0.  Each work item computes the bounding box
-// of its targets once (a wave reduction); every staged source tile then checks, one source per lane
-// and one ballot, whether all its sources lie outside that box by ClearGap on some axis.  Such a tile
-// has r2 > 0 for every pair, and runs the micro-kernel's unmasked form (eval<R, false>), which gives
-// the same results without the compare and select of each pair.  In a U-list only the tiles of the
-// target leaf itself and of touching neighbours near it need the mask.
+// Coincidence mask: source tiles clear of the item's target box run the unmasked pair form
+// (p2p_clear.hpp); in a U-list only the tiles of the target leaf and of nearby parts of touching
+// neighbours keep the mask.
 //
 // make_uker_ops<UK>() bundles every launch entry of one micro-kernel (LDS and scalar-broadcast
 // item kernels for PREC x T, their occupancy queries, record staging, dimensions) into a UkerOps
@@ -30,6 +27,7 @@
 
 #include <type_traits>
 
+#include "p2p_clear.hpp"
 #include "p2p_internal.hpp"
 #include "p2p_queue.hpp"
 #include "p2p_sgpr.hpp"
@@ -59,32 +57,6 @@ namespace stkp2p {
 #else
 #define STKP2P_ITEM_ATTR
 #endif
-
-// one pair through the micro-kernel, without the coincidence mask when MASK is false and the
-// micro-kernel offers that form
-template <class K, bool MASK, class R>
-__device__ __forceinline__ void eval_pair(R *u, R dx, R dy, R dz, const R *f) {
-    if constexpr (HasUnmaskedEval<K>::value)
-        K::template eval<R, MASK>(u, dx, dy, dz, f);
-    else
-        K::template eval<R>(u, dx, dy, dz, f);
-}
-
-// a wave-uniform double held in scalar registers (every lane has the same value)
-__device__ __forceinline__ double uniform_f64(double v) {
-    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
-    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
-    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
-}
-
-// Minimum separation, per axis, between a source and the bounding box of an item's targets for
-// which every pair of the source is known to have r2 > 0 in the pair arithmetic: FP64, |dx| >=
-// 1e-150 keeps dx^2 above the smallest normal double; mixed, |dx| >= 4e-18 (after the rounding of
-// the box edge, at least 2e-18) keeps the FP32 dx^2 above the smallest normal float.
-template <int PREC> struct ClearGap;
-template <> struct ClearGap<0> { static constexpr double value = 1e-150; };
-template <> struct ClearGap<1> { static constexpr double value = 4e-18; };
 
 template <int UK, int PREC, int T>
 __global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemArgs a) {
@@ -141,36 +113,11 @@ __global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemA
                 tlz[t] = (float)(tz[t] - (double)thz[t]);
             }
         }
-        // bounding box of the item's targets (coordinates as the pairs use them); a source tile with
-        // every source outside it by ClearGap on some axis has no coincident pair and runs unmasked
+        // target box for the unmasked shortcut on source tiles clear of it (p2p_clear.hpp)
         constexpr bool CLEAR = HasUnmaskedEval<K>::value && !HILO;
-        double bmin[3], bmax[3];
-        if constexpr (CLEAR) {
-            bmin[0] = bmax[0] = tx[0];
-            bmin[1] = bmax[1] = ty[0];
-            bmin[2] = bmax[2] = tz[0];
-#pragma unroll
-            for (int t = 1; t < T; ++t) {
-                bmin[0] = fmin(bmin[0], tx[t]);
-                bmax[0] = fmax(bmax[0], tx[t]);
-                bmin[1] = fmin(bmin[1], ty[t]);
-                bmax[1] = fmax(bmax[1], ty[t]);
-                bmin[2] = fmin(bmin[2], tz[t]);
-                bmax[2] = fmax(bmax[2], tz[t]);
-            }
-            for (int off = 1; off < kWave; off <<= 1) {
-#pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    bmin[d] = fmin(bmin[d], __shfl_xor(bmin[d], off));
-                    bmax[d] = fmax(bmax[d], __shfl_xor(bmax[d], off));
-                }
-            }
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                bmin[d] = uniform_f64(bmin[d] - ClearGap<PREC>::value);
-                bmax[d] = uniform_f64(bmax[d] + ClearGap<PREC>::value);
-            }
-        }
+        TargetBox<PREC> box;
+        if constexpr (CLEAR)
+            box.build(tx, ty, tz);
         double acc[T][KT];
 #pragma unroll
         for (int t = 0; t < T; ++t)
@@ -204,8 +151,7 @@ __global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemA
                                 vin[k] *= lam;
                     }
                     if constexpr (CLEAR)
-                        near = !(x < bmin[0] || x > bmax[0] || y < bmin[1] || y > bmax[1] || z < bmin[2] ||
-                                 z > bmax[2]);
+                        near = box.near(x, y, z);
                     double vst[KL];
                     K::stage(vin, vst);
                     if constexpr (HILO) {

@@ -11,12 +11,17 @@
 //
 // Tiles that spread a target over several lanes (lsplit > 0) read records with per-lane vector
 // loads instead, since their lanes walk different sources.
+//
+// Coincidence mask: the uniform path walks each range in chunks of 64 records; a chunk whose
+// sources are all clear of the item's target box (checked one source per lane, one ballot) runs the
+// unmasked pair form (p2p_clear.hpp).
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
 
+#include "p2p_clear.hpp"
 #include "p2p_internal.hpp"
 #include "p2p_queue.hpp"
 #include "ukernels.hpp"
@@ -103,6 +108,10 @@ __global__ void __launch_bounds__(kWave) p2p_items_sgpr_kernel(ItemArgs a, const
             ty[t] = a.tc.p[idx * a.tc.s + a.tc.fs] * lam;
             tz[t] = a.tc.p[idx * a.tc.s + 2 * a.tc.fs] * lam;
         }
+        constexpr bool CLEAR = HasUnmaskedEval<K>::value;
+        TargetBox<PREC> box;
+        if constexpr (CLEAR)
+            box.build(tx, ty, tz);
         double acc[T][KT];
 #pragma unroll
         for (int t = 0; t < T; ++t)
@@ -112,7 +121,8 @@ __global__ void __launch_bounds__(kWave) p2p_items_sgpr_kernel(ItemArgs a, const
 
         // one source record against the lane's T targets; P is the record pointer type (scalar
         // address space for the uniform path, generic for the per-lane path)
-        auto pair_fp64 = [&](auto p) {
+        auto pair_fp64 = [&](auto p, auto unmasked) {
+            constexpr bool MASK = !decltype(unmasked)::value;
             const double sx = p[0], sy = p[1], sz = p[2];
             double f[KL];
 #pragma unroll
@@ -120,52 +130,74 @@ __global__ void __launch_bounds__(kWave) p2p_items_sgpr_kernel(ItemArgs a, const
                 f[k] = p[3 + k];
 #pragma unroll
             for (int t = 0; t < T; ++t)
-                K::template eval<double>(acc[t], tx[t] - sx, ty[t] - sy, tz[t] - sz, f);
+                eval_pair<K, MASK>(acc[t], tx[t] - sx, ty[t] - sy, tz[t] - sz, f);
         };
 
         for (int r = item.rng_begin; r < item.rng_end; ++r) {
             const int2 rg = a.ranges[r];
             if (ls == 0) {
                 cdouble_t *R = (cdouble_t *)(unsigned long long)(rbase);
-                if (PREC == 0) {
+                // records [c0, c1) of the range, masked or not
+                auto chunk = [&](int c0, int c1, auto unmasked) {
+                    constexpr bool MASK = !decltype(unmasked)::value;
+                    if (PREC == 0) {
 #pragma unroll 4
-                    for (int j = rg.x; j < rg.y; ++j)
-                        pair_fp64(R + (long long)j * L::WORDS);
-                } else {
-                    for (int j0 = rg.x; j0 < rg.y; j0 += kMixedAccBlock) {
-                        const int j1 = (rg.y - j0 < kMixedAccBlock) ? rg.y : j0 + kMixedAccBlock;
-                        float part[T][KT];
-#pragma unroll
-                        for (int t = 0; t < T; ++t)
-#pragma unroll
-                            for (int k = 0; k < KT; ++k)
-                                part[t][k] = 0.0f;
-#pragma unroll 4
-                        for (int j = j0; j < j1; ++j) {
-                            cdouble_t *p = R + (long long)j * L::WORDS;
-                            const double sx = p[0], sy = p[1], sz = p[2];
-                            cfloat_t *pf = (cfloat_t *)(p + 3);
-                            float f[KL];
-#pragma unroll
-                            for (int k = 0; k < KL; ++k)
-                                f[k] = pf[k];
+                        for (int j = c0; j < c1; ++j)
+                            pair_fp64(R + (long long)j * L::WORDS, unmasked);
+                    } else {
+                        for (int j0 = c0; j0 < c1; j0 += kMixedAccBlock) {
+                            const int j1 = (c1 - j0 < kMixedAccBlock) ? c1 : j0 + kMixedAccBlock;
+                            float part[T][KT];
 #pragma unroll
                             for (int t = 0; t < T; ++t)
-                                K::template eval<float>(part[t], (float)(tx[t] - sx), (float)(ty[t] - sy),
-                                                        (float)(tz[t] - sz), f);
+#pragma unroll
+                                for (int k = 0; k < KT; ++k)
+                                    part[t][k] = 0.0f;
+#pragma unroll 4
+                            for (int j = j0; j < j1; ++j) {
+                                cdouble_t *p = R + (long long)j * L::WORDS;
+                                const double sx = p[0], sy = p[1], sz = p[2];
+                                cfloat_t *pf = (cfloat_t *)(p + 3);
+                                float f[KL];
+#pragma unroll
+                                for (int k = 0; k < KL; ++k)
+                                    f[k] = pf[k];
+#pragma unroll
+                                for (int t = 0; t < T; ++t)
+                                    eval_pair<K, MASK>(part[t], (float)(tx[t] - sx), (float)(ty[t] - sy),
+                                                       (float)(tz[t] - sz), f);
+                            }
+#pragma unroll
+                            for (int t = 0; t < T; ++t)
+#pragma unroll
+                                for (int k = 0; k < KT; ++k)
+                                    acc[t][k] += (double)part[t][k];
                         }
-#pragma unroll
-                        for (int t = 0; t < T; ++t)
-#pragma unroll
-                            for (int k = 0; k < KT; ++k)
-                                acc[t][k] += (double)part[t][k];
                     }
+                };
+                // chunks of 64 records (a multiple of kMixedAccBlock, so the FP32 partial sums group
+                // the same sources as a walk of the whole range)
+                for (int c0 = rg.x; c0 < rg.y; c0 += kWave) {
+                    const int c1 = (rg.y - c0 < kWave) ? rg.y : c0 + kWave;
+                    bool clear = false;
+                    if constexpr (CLEAR) {
+                        bool near = false;
+                        if (c0 + lane < c1) {
+                            const double *p = rbase + (long long)(c0 + lane) * L::WORDS;
+                            near = box.near(p[0], p[1], p[2]);
+                        }
+                        clear = !__any(near);
+                    }
+                    if (clear)
+                        chunk(c0, c1, std::true_type{});
+                    else
+                        chunk(c0, c1, std::false_type{});
                 }
             } else {
                 // per-lane sources: lane sub takes every split-th source of the range
                 if (PREC == 0) {
                     for (int j = rg.x + sub; j < rg.y; j += split)
-                        pair_fp64(rbase + (long long)j * L::WORDS);
+                        pair_fp64(rbase + (long long)j * L::WORDS, std::false_type{});
                 } else {
                     for (int j0 = rg.x; j0 < rg.y; j0 += kMixedAccBlock) {
                         const int j1 = (rg.y - j0 < kMixedAccBlock) ? rg.y : j0 + kMixedAccBlock;

@@ -42,7 +42,10 @@ def run(uk, prec, bc, x, cap, seed):
     plan = S.Plan.from_tree(tr, uk, prec, tune=dict(broadcast=bc))
     out = zeros(tr.n, O.trg_dim(uk))
     d_x = to_dev(xs)
-    plan.execute(d_x, to_dev(v), d_x, out)
+    if uk == S.UKER.STOKES_PVEL_FUSED:
+        plan.execute(d_x, to_dev(v[:, :4]), d_x, out, src_value2=to_dev(v[:, 4:]))
+    else:
+        plan.execute(d_x, to_dev(v), d_x, out)
     ref = O.ranges(uk, xs, v, xs, tr.leaf_begin, tr.leaf_end, tr.ulist_ptr, tr.rng_begin, tr.rng_end)
     return host(out), ref, tr
 
@@ -54,6 +57,22 @@ def test_family_ulist_vs_oracle(uk, prec, bc):
     say(f"family U-list uker={uk} prec={prec} broadcast={bc}")
     x = clustered_points(700 + uk)
     out, ref, tr = run(uk, prec, bc, x, 120, 800 + uk)
+    assert tr.max_level >= 5
+    assert np.all(np.isfinite(out))
+    l2, mx = O.max_error(out, ref)
+    tol = FP64_TOL if prec == 0 else MIXED_TOL
+    assert l2 < tol and mx < tol, (l2, mx)
+
+
+@pytest.mark.parametrize("bc", [0, 1], ids=["lds", "scalar"])
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("uk", range(14))
+def test_core_ulist_clustered_vs_oracle(uk, prec, bc):
+    # the same deep clustered trees with pairs 1e-6 apart for the core kernels: the Laplace
+    # gradient-of-gradient kernels once overflowed FP32 there (a literal 1/r^7)
+    say(f"core clustered U-list uker={uk} prec={prec} broadcast={bc}")
+    x = clustered_points(1700 + uk)
+    out, ref, tr = run(uk, prec, bc, x, 120, 1800 + uk)
     assert tr.max_level >= 5
     assert np.all(np.isfinite(out))
     l2, mx = O.max_error(out, ref)
