@@ -51,6 +51,8 @@ def parse():
                     help="run one variant only (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device copy of known size (PMC calibration) and exit")
+    ap.add_argument("--valu-rates", action="store_true",
+                    help="run stkfmm_amd/valu_rates (VALU issue rate per instruction kind) and exit")
     return ap.parse_args()
 
 
@@ -373,6 +375,12 @@ def main():
                                    "(multi-GPU sharding is a design note, not a measured mode)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.valu_rates:
+        import subprocess
+        exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "stkfmm_amd", "valu_rates")
+        if not os.path.exists(exe):
+            sys.exit("stkfmm_amd/valu_rates is not built (make -C stkfmm_amd/csrc)")
+        sys.exit(subprocess.call([exe]))
     import torch
 
     import stkfmm_amd as S

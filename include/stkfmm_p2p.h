@@ -360,7 +360,9 @@ int stkfmm_wall_eval_device(unsigned kernel, int precision, int nSL, const doubl
  * has no derivatives); callers: the torch.autograd.Function in the Python binding.
  * Device AoS arrays.  Outputs are OVERWRITTEN: g_src_value [nSrc*ks], g_src_coord [nSrc*3],
  * g_trg_coord [nTrg*3]; any of them may be NULL to skip it.  For the fused kernel src_value holds
- * the 13 values [f(3), TrD, D(9)] of each point. */
+ * the 13 values [f(3), TrD, D(9)] of each point.  One pass over the pairs; deterministic.  The
+ * environment variable STKFMM_VJP_WAVES_PER_CU (default 32) sets how many waves per compute unit the
+ * pass is split into (more: better balance, larger partial-sum workspace). */
 int stkfmm_p2p_vjp_device(int uker, int precision, int nSrc, const double *d_srcCoord, const double *d_srcValue,
                           int nTrg, const double *d_trgCoord, const double *d_w, double *d_g_src_value,
                           double *d_g_src_coord, double *d_g_trg_coord, void *stream);

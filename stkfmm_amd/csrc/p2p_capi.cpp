@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -1060,21 +1061,39 @@ extern "C" int stkfmm_p2p_vjp_device(int uker, int precision, int nSrc, const do
     if (rc)
         return rc;
     const int ks = uker_ks(uker);
-    // enough workgroups to fill the device: about 8 per CU, chunks of at least 256 entities
-    auto chunks = [&](int n_lane, int n_other) {
-        const long long lanes_wg = std::max(1, (n_lane + kWave - 1) / kWave);
-        long long c = (8LL * n_cu + lanes_wg - 1) / lanes_wg;
-        c = std::min<long long>(c, std::max(1, n_other / 256));
-        return (int)std::max(1LL, c);
-    };
-    const int nct = chunks(nTrg, nSrc), ncs = chunks(nSrc, nTrg);
+    // about 32 waves per CU over a (target blocks) x (source chunks) grid, close to square (measured on
+    // MI355X, 65,536^2 PVel: 8 waves per CU 11.7 ms FP64 / 9.8 ms mixed, 24: 11.3 / 8.5, 48: 11.1 /
+    // 8.4; T = 1 slower); partials kept within ~2 GiB by fewer, larger blocks for large point counts
+    VjpLayout l;
+    l.T = (nTrg >= 16 * kWave) ? 2 : 1;
+    const long long ttiles = std::max(1, (nTrg + kWave * l.T - 1) / (kWave * l.T));
+    const long long stiles = std::max(1, (nSrc + kSrcTile - 1) / kSrcTile);
+    long long per_cu = 32;
+    if (const char *e = std::getenv("STKFMM_VJP_WAVES_PER_CU")) // tuning and tests: other layouts
+        per_cu = std::max(1L, std::strtol(e, nullptr, 10));
+    const long long waves = per_cu * n_cu;
+    const long long root = (long long)std::ceil(std::sqrt((double)waves));
+    long long nc = std::min(stiles, root);
+    long long nb = std::min(ttiles, (waves + nc - 1) / nc);
+    nc = std::min(stiles, (waves + nb - 1) / nb);
+    const long long budget = (2LL << 30) / 8;
+    while ((nc * 3 * nTrg + nb * (3 + ks) * (long long)nSrc) > budget && (nb > 1 || nc > 1)) {
+        if (nb >= nc)
+            nb = (nb + 1) / 2;
+        else
+            nc = (nc + 1) / 2;
+    }
+    l.trg_tiles_per_block = (int)((ttiles + nb - 1) / nb);
+    l.n_blocks = (int)((ttiles + l.trg_tiles_per_block - 1) / l.trg_tiles_per_block);
+    l.src_tiles_per_chunk = (int)((stiles + nc - 1) / nc);
+    l.n_chunks = (int)((stiles + l.src_tiles_per_chunk - 1) / l.src_tiles_per_chunk);
     std::lock_guard<std::mutex> lk(g_mu);
     double *partial = nullptr;
-    const size_t need = std::max((size_t)nct * 3 * nTrg, (size_t)ncs * (3 + ks) * nSrc);
-    if ((rc = ws_get(4, need, &partial)))
+    const size_t need = (size_t)l.n_chunks * 3 * nTrg + (size_t)l.n_blocks * (3 + ks) * nSrc;
+    if ((rc = ws_get(4, std::max<size_t>(need, 1), &partial)))
         return rc;
     VjpArgs a{d_srcCoord, d_srcValue, d_trgCoord, d_w, nSrc, nTrg, uker_scale(uker),
               d_g_src_value, d_g_src_coord, d_g_trg_coord};
-    HIP_TRY(launch_vjp(uker, precision, a, partial, nct, ncs, (hipStream_t)stream), "vjp launch");
+    HIP_TRY(launch_vjp(uker, precision, a, l, partial, (hipStream_t)stream), "vjp launch");
     return STKFMM_OK;
 }

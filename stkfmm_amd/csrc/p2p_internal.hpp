@@ -161,9 +161,13 @@ hipError_t launch_wall_prep_rpy(int n, const double *sc, const double *sv, const
 hipError_t launch_wall_zero(double *p, long long n, hipStream_t s);
 hipError_t launch_wall_assemble_stokes(int m, const double *tc, const WallStokes &w, double *out, hipStream_t s);
 hipError_t launch_wall_assemble_rpy(int m, const double *tc, const WallRpy &w, double *out, hipStream_t s);
-// chunk counts: the target pass splits the sources into nchunk_t chunks, the source pass splits the
-// targets into nchunk_s chunks; partial must hold max(nchunk_t*3*n_trg, nchunk_s*(3+KS)*n_src)
-hipError_t launch_vjp(int uk, int prec, const VjpArgs &a, double *partial, int nchunk_t, int nchunk_s,
-                      hipStream_t s);
-
+// adjoint work decomposition: T targets per lane; the grid is n_blocks target blocks (of
+// trg_tiles_per_block tiles of 64 T targets) x n_chunks source chunks (of src_tiles_per_chunk tiles of
+// 64 sources); partial holds n_chunks * 3 * n_trg + n_blocks * (3 + KS) * n_src doubles
+struct VjpLayout {
+    int T;
+    int trg_tiles_per_block, n_blocks;
+    int src_tiles_per_chunk, n_chunks;
+};
+hipError_t launch_vjp(int uk, int prec, const VjpArgs &a, const VjpLayout &l, double *partial, hipStream_t s);
 } // namespace stkp2p

@@ -167,61 +167,53 @@ double uker_scale(int uk) {
 
 // ---- adjoint dispatch -------------------------------------------------------------------------
 
-template <int UK, int PREC>
-static hipError_t launch_vjp_t(const VjpArgs &a, double *partial, int nchunk_t, int nchunk_s, hipStream_t s) {
+template <int UK, int PREC, int T>
+static hipError_t launch_vjp_t(const VjpArgs &a, const VjpLayout &l, double *partial, hipStream_t s) {
     using V = typename VjpOf<UK>::type;
-    constexpr int KS = V::KS;
-    VjpPass p{a.sc, a.sv, a.tc, a.w, a.n_src, a.n_trg, 0, partial};
-    if (a.g_tc && a.n_trg > 0 && a.n_src > 0) {
-        p.chunk = (a.n_src + nchunk_t - 1) / nchunk_t;
-        const int nc = (a.n_src + p.chunk - 1) / p.chunk;
-        hipLaunchKernelGGL((vjp_target_kernel<V, PREC>), dim3((a.n_trg + kWave - 1) / kWave, nc), dim3(kWave), 0,
-                           s, p);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess)
-            return e;
-        hipLaunchKernelGGL(vjp_reduce_kernel, dim3((a.n_trg + 255) / 256), dim3(256), 0, s, (const double *)partial,
-                           a.n_trg, 3, nc, 0, 3, a.g_tc, 3, 0, a.scale);
-        e = hipGetLastError();
-        if (e != hipSuccess)
-            return e;
-    } else if (a.g_tc && a.n_trg > 0) {
-        zero_fill(a.g_tc, 3LL * a.n_trg, s);
+    constexpr int KS = V::KS, ND = 3 + KS;
+    if (a.n_trg == 0 || a.n_src == 0) {
+        if (a.g_tc)
+            zero_fill(a.g_tc, 3LL * a.n_trg, s);
+        if (a.g_sc)
+            zero_fill(a.g_sc, 3LL * a.n_src, s);
+        if (a.g_sv)
+            zero_fill(a.g_sv, (long long)KS * a.n_src, s);
+        return hipGetLastError();
     }
-    if ((a.g_sc || a.g_sv) && a.n_src > 0 && a.n_trg > 0) {
-        p.chunk = (a.n_trg + nchunk_s - 1) / nchunk_s;
-        const int nc = (a.n_trg + p.chunk - 1) / p.chunk;
-        hipLaunchKernelGGL((vjp_source_kernel<V, PREC>), dim3((a.n_src + kWave - 1) / kWave, nc), dim3(kWave), 0,
-                           s, p);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess)
-            return e;
-        const dim3 g((a.n_src + 255) / 256);
-        hipLaunchKernelGGL(vjp_reduce_kernel, g, dim3(256), 0, s, (const double *)partial, a.n_src, 3 + KS, nc, 0, 3,
-                           a.g_sc, 3, 0, a.scale);
-        hipLaunchKernelGGL(vjp_reduce_kernel, g, dim3(256), 0, s, (const double *)partial, a.n_src, 3 + KS, nc, 3,
-                           3 + KS, a.g_sv, KS, 0, a.scale);
-        e = hipGetLastError();
-        if (e != hipSuccess)
-            return e;
-    } else if (a.n_src > 0) {
-        zero_fill(a.g_sc, 3LL * a.n_src, s);
-        zero_fill(a.g_sv, (long long)KS * a.n_src, s);
-    }
+    double *tpart = partial, *spart = partial + (long long)l.n_chunks * 3 * a.n_trg;
+    VjpPass p{a.sc, a.sv, a.tc, a.w, a.n_src, a.n_trg, l.trg_tiles_per_block, l.src_tiles_per_chunk, tpart, spart};
+    hipLaunchKernelGGL((vjp_pair_kernel<V, PREC, T>), dim3(l.n_blocks, l.n_chunks), dim3(kWave), 0, s, p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    const dim3 gt((a.n_trg + 255) / 256), gs((a.n_src + 255) / 256);
+    if (a.g_tc)
+        hipLaunchKernelGGL(vjp_reduce_kernel, gt, dim3(256), 0, s, (const double *)tpart, a.n_trg, 3, l.n_chunks, 0,
+                           3, a.g_tc, 3, 0, a.scale);
+    if (a.g_sc)
+        hipLaunchKernelGGL(vjp_reduce_kernel, gs, dim3(256), 0, s, (const double *)spart, a.n_src, ND, l.n_blocks, 0,
+                           3, a.g_sc, 3, 0, a.scale);
+    if (a.g_sv)
+        hipLaunchKernelGGL(vjp_reduce_kernel, gs, dim3(256), 0, s, (const double *)spart, a.n_src, ND, l.n_blocks, 3,
+                           ND, a.g_sv, KS, 0, a.scale);
     return hipGetLastError();
+}
+
+template <int UK, int PREC>
+static hipError_t launch_vjp_p(const VjpArgs &a, const VjpLayout &l, double *partial, hipStream_t s) {
+    return (l.T == 1) ? launch_vjp_t<UK, PREC, 1>(a, l, partial, s) : launch_vjp_t<UK, PREC, 2>(a, l, partial, s);
 }
 
 bool uker_has_vjp(int uk) { return uk == 7 || uk == 8 || uk == 11 || uk == 13; }
 
-hipError_t launch_vjp(int uk, int prec, const VjpArgs &a, double *partial, int nchunk_t, int nchunk_s,
-                      hipStream_t s) {
-    if (nchunk_t < 1 || nchunk_s < 1)
+hipError_t launch_vjp(int uk, int prec, const VjpArgs &a, const VjpLayout &l, double *partial, hipStream_t s) {
+    if (l.T != 1 && l.T != 2)
         return hipErrorInvalidValue;
     switch (uk) {
-    case 7: return prec ? launch_vjp_t<7, 1>(a, partial, nchunk_t, nchunk_s, s) : launch_vjp_t<7, 0>(a, partial, nchunk_t, nchunk_s, s);
-    case 8: return prec ? launch_vjp_t<8, 1>(a, partial, nchunk_t, nchunk_s, s) : launch_vjp_t<8, 0>(a, partial, nchunk_t, nchunk_s, s);
-    case 11: return prec ? launch_vjp_t<11, 1>(a, partial, nchunk_t, nchunk_s, s) : launch_vjp_t<11, 0>(a, partial, nchunk_t, nchunk_s, s);
-    case 13: return prec ? launch_vjp_t<13, 1>(a, partial, nchunk_t, nchunk_s, s) : launch_vjp_t<13, 0>(a, partial, nchunk_t, nchunk_s, s);
+    case 7: return prec ? launch_vjp_p<7, 1>(a, l, partial, s) : launch_vjp_p<7, 0>(a, l, partial, s);
+    case 8: return prec ? launch_vjp_p<8, 1>(a, l, partial, s) : launch_vjp_p<8, 0>(a, l, partial, s);
+    case 11: return prec ? launch_vjp_p<11, 1>(a, l, partial, s) : launch_vjp_p<11, 0>(a, l, partial, s);
+    case 13: return prec ? launch_vjp_p<13, 1>(a, l, partial, s) : launch_vjp_p<13, 0>(a, l, partial, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -7,13 +7,19 @@
 // The per-pair formulas are differentiated by hand from the forward micro-kernels (ukernels.hpp).
 // Coincident pairs contribute zero (the forward is masked there).  The reference has no adjoint.
 //
-// Two passes, each a chunked direct sum with a lane per entity:
-//   target pass: lane = target, sources staged in LDS -> partial g_trg_coord
-//   source pass: lane = source, targets (coords + w) staged in LDS -> partial g_src_coord, g_src_value
-// then a reduction over chunks (deterministic order) writes scale * sum (overwrite).
+// One pass evaluates each pair once (vjp_pair_kernel): a wave holds T targets per lane and walks a
+// chunk of source tiles staged in LDS.  Within a tile, at step k lane i pairs its targets with
+// source slot (i + k) mod 64 (a conflict-free lane-indexed LDS read); the target gradient stays in
+// the lane, and the source-side sums (position gradient, value gradient) travel with their source:
+// after every step they move one lane down the wave (a DPP rotation), so after 64 steps each has
+// met every target of the tile and is back in the lane of its slot.  The grid is (target blocks) x
+// (source chunks); every (source chunk, target) and (target block, source) partial is written by
+// one wave in a fixed order and the partials are summed in a fixed order -- deterministic.
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "p2p_internal.hpp"
 #include "ukernels.hpp"
@@ -126,123 +132,131 @@ template <> struct VjpOf<8> { using type = VjpStokesDoublePVel; };
 template <> struct VjpOf<11> { using type = VjpRpyULapU; };
 template <> struct VjpOf<13> { using type = VjpStokesPVelFused; };
 
-// partial layout: [chunk][dim][n_lane]
+// lane i receives the value of lane i + 1 (lane 63 that of lane 0): DPP wave_rol:1
+__device__ __forceinline__ float rotate_down(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x134, 0xF, 0xF,
+                                                                 false));
+}
+__device__ __forceinline__ double rotate_down(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, 0x134, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), 0x134, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// tpart: [n_chunks][3][n_trg] target position gradient over the sources of one chunk
+// spart: [n_blocks][3 + KS][n_src] source position and value gradients over the targets of one block
 struct VjpPass {
-    const double *sc, *sv, *tc, *w;
+    const double *sc, *sv, *tc, *w;  // AoS
     int n_src, n_trg;
-    int chunk;          // entities of the other side per chunk
-    double *partial;
+    int trg_tiles_per_block;         // target tiles (64 T targets) per block
+    int src_tiles_per_chunk;         // source tiles (64 sources) per chunk
+    double *tpart;
+    double *spart;
 };
 
-// lane = target; sources [c*chunk, (c+1)*chunk) staged in LDS; partial dims 3
-template <class V, int PREC>
-__global__ void __launch_bounds__(kWave) vjp_target_kernel(VjpPass a) {
+template <class V, int PREC, int T>
+__global__ void __launch_bounds__(kWave) vjp_pair_kernel(VjpPass a) {
     using CT = typename std::conditional<PREC == 0, double, float>::type;
-    constexpr int KS = V::KS, KT = V::KT;
+    constexpr int KS = V::KS, KT = V::KT, ND = 3 + KS, TT = kWave * T;
     __shared__ double s_x[kSrcTile], s_y[kSrcTile], s_z[kSrcTile];
     __shared__ CT s_v[KS][kSrcTile];
     const int lane = threadIdx.x;
-    const int t = blockIdx.x * kWave + lane;
-    const int tt = (t < a.n_trg) ? t : a.n_trg - 1;
-    const double tx = a.tc[3LL * tt], ty = a.tc[3LL * tt + 1], tz = a.tc[3LL * tt + 2];
-    CT w[KT];
-#pragma unroll
-    for (int k = 0; k < KT; ++k)
-        w[k] = (CT)a.w[(long long)KT * tt + k];
-    double g[3] = {0.0, 0.0, 0.0};
-    const int s0 = blockIdx.y * a.chunk;
-    const int s1 = (s0 + a.chunk < a.n_src) ? s0 + a.chunk : a.n_src;
-    for (int base = s0; base < s1; base += kSrcTile) {
-        const int n = (s1 - base < kSrcTile) ? s1 - base : kSrcTile;
-        __syncthreads();
-        if (lane < n) {
-            const long long s = base + lane;
+    const int n_trg_tiles = (a.n_trg + TT - 1) / TT;
+    const int n_src_tiles = (a.n_src + kSrcTile - 1) / kSrcTile;
+    const int tt0 = blockIdx.x * a.trg_tiles_per_block;
+    const int tt1 = min(tt0 + a.trg_tiles_per_block, n_trg_tiles);
+    const int st0 = blockIdx.y * a.src_tiles_per_chunk;
+    const int st1 = min(st0 + a.src_tiles_per_chunk, n_src_tiles);
+    double *tpart = a.tpart + (long long)blockIdx.y * 3 * a.n_trg;
+    double *spart = a.spart + (long long)blockIdx.x * ND * a.n_src;
+
+    for (int st = st0; st < st1; ++st) {
+        const int sbase = st * kSrcTile;
+        const int ns = min(kSrcTile, a.n_src - sbase);
+        __syncthreads(); // every lane has read the previous tile
+        {
+            // slot lane: its source, or (past the end) a copy of the last one with zero values, which
+            // contributes nothing to the targets
+            const long long s = sbase + min(lane, ns - 1);
             s_x[lane] = a.sc[3 * s];
             s_y[lane] = a.sc[3 * s + 1];
             s_z[lane] = a.sc[3 * s + 2];
 #pragma unroll
             for (int k = 0; k < KS; ++k)
-                s_v[k][lane] = (CT)a.sv[KS * s + k];
+                s_v[k][lane] = (lane < ns) ? (CT)a.sv[KS * s + k] : CT(0);
         }
         __syncthreads();
-        CT gp[3] = {CT(0), CT(0), CT(0)};
-        for (int j = 0; j < n; ++j) {
-            CT f[KS], gf[KS];
-#pragma unroll
-            for (int k = 0; k < KS; ++k) {
-                f[k] = s_v[k][j];
-                gf[k] = CT(0);
-            }
-            V::template vjp<CT>((CT)(tx - s_x[j]), (CT)(ty - s_y[j]), (CT)(tz - s_z[j]), f, w, gp, gf);
-        }
-        g[0] += (double)gp[0];
-        g[1] += (double)gp[1];
-        g[2] += (double)gp[2];
-    }
-    if (t < a.n_trg) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-            a.partial[((long long)blockIdx.y * 3 + d) * a.n_trg + t] = g[d];
-    }
-}
-
-// lane = source; targets [c*chunk, (c+1)*chunk) staged in LDS; partial dims 3 + KS
-template <class V, int PREC>
-__global__ void __launch_bounds__(kWave) vjp_source_kernel(VjpPass a) {
-    using CT = typename std::conditional<PREC == 0, double, float>::type;
-    constexpr int KS = V::KS, KT = V::KT, ND = 3 + KS;
-    __shared__ double s_x[kSrcTile], s_y[kSrcTile], s_z[kSrcTile];
-    __shared__ CT s_w[KT][kSrcTile];
-    const int lane = threadIdx.x;
-    const int s = blockIdx.x * kWave + lane;
-    const int ss = (s < a.n_src) ? s : a.n_src - 1;
-    const double sx = a.sc[3LL * ss], sy = a.sc[3LL * ss + 1], sz = a.sc[3LL * ss + 2];
-    CT f[KS];
-#pragma unroll
-    for (int k = 0; k < KS; ++k)
-        f[k] = (CT)a.sv[(long long)KS * ss + k];
-    double g[ND];
-#pragma unroll
-    for (int d = 0; d < ND; ++d)
-        g[d] = 0.0;
-    const int t0 = blockIdx.y * a.chunk;
-    const int t1 = (t0 + a.chunk < a.n_trg) ? t0 + a.chunk : a.n_trg;
-    for (int base = t0; base < t1; base += kSrcTile) {
-        const int n = (t1 - base < kSrcTile) ? t1 - base : kSrcTile;
-        __syncthreads();
-        if (lane < n) {
-            const long long t = base + lane;
-            s_x[lane] = a.tc[3 * t];
-            s_y[lane] = a.tc[3 * t + 1];
-            s_z[lane] = a.tc[3 * t + 2];
-#pragma unroll
-            for (int k = 0; k < KT; ++k)
-                s_w[k][lane] = (CT)a.w[KT * t + k];
-        }
-        __syncthreads();
-        CT gr[3] = {CT(0), CT(0), CT(0)};
-        CT gf[KS];
-#pragma unroll
-        for (int k = 0; k < KS; ++k)
-            gf[k] = CT(0);
-        for (int j = 0; j < n; ++j) {
-            CT w[KT];
-#pragma unroll
-            for (int k = 0; k < KT; ++k)
-                w[k] = s_w[k][j];
-            V::template vjp<CT>((CT)(s_x[j] - sx), (CT)(s_y[j] - sy), (CT)(s_z[j] - sz), f, w, gr, gf);
-        }
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-            g[d] -= (double)gr[d];
-#pragma unroll
-        for (int k = 0; k < KS; ++k)
-            g[3 + k] += (double)gf[k];
-    }
-    if (s < a.n_src) {
+        double sacc[ND];
 #pragma unroll
         for (int d = 0; d < ND; ++d)
-            a.partial[((long long)blockIdx.y * ND + d) * a.n_src + s] = g[d];
+            sacc[d] = 0.0;
+
+        for (int tt = tt0; tt < tt1; ++tt) {
+            // targets tt*TT + lane + 64 t; past the end: a copy of the last one with a zero cotangent
+            double tx[T], ty[T], tz[T];
+            CT w[T][KT];
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const int i = tt * TT + lane + kWave * t;
+                const long long ic = (i < a.n_trg) ? i : a.n_trg - 1;
+                tx[t] = a.tc[3 * ic];
+                ty[t] = a.tc[3 * ic + 1];
+                tz[t] = a.tc[3 * ic + 2];
+#pragma unroll
+                for (int k = 0; k < KT; ++k)
+                    w[t][k] = (i < a.n_trg) ? (CT)a.w[KT * ic + k] : CT(0);
+            }
+            CT tg[T][3];
+            CT rs[ND]; // source-side sums of the source currently in this lane
+#pragma unroll
+            for (int t = 0; t < T; ++t)
+                tg[t][0] = tg[t][1] = tg[t][2] = CT(0);
+#pragma unroll
+            for (int d = 0; d < ND; ++d)
+                rs[d] = CT(0);
+            for (int k = 0; k < kSrcTile; ++k) {
+                const int j = (lane + k) & (kSrcTile - 1);
+                const double sx = s_x[j], sy = s_y[j], sz = s_z[j];
+                CT f[KS];
+#pragma unroll
+                for (int q = 0; q < KS; ++q)
+                    f[q] = s_v[q][j];
+#pragma unroll
+                for (int t = 0; t < T; ++t) {
+                    CT gr[3] = {CT(0), CT(0), CT(0)};
+                    V::template vjp<CT>((CT)(tx[t] - sx), (CT)(ty[t] - sy), (CT)(tz[t] - sz), f, w[t], gr, rs + 3);
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        tg[t][d] += gr[d];
+                        rs[d] -= gr[d];
+                    }
+                }
+#pragma unroll
+                for (int d = 0; d < ND; ++d)
+                    rs[d] = rotate_down(rs[d]);
+            }
+            // 64 rotations: rs is back in the lane of its slot
+#pragma unroll
+            for (int d = 0; d < ND; ++d)
+                sacc[d] += (double)rs[d];
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const int i = tt * TT + lane + kWave * t;
+                if (i < a.n_trg) {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        double *p = tpart + (long long)d * a.n_trg + i;
+                        *p = ((st == st0) ? 0.0 : *p) + (double)tg[t][d];
+                    }
+                }
+            }
+        }
+        if (lane < ns) {
+#pragma unroll
+            for (int d = 0; d < ND; ++d)
+                spart[(long long)d * a.n_src + sbase + lane] = sacc[d];
+        }
     }
 }
 

@@ -29,6 +29,28 @@ def test_vjp_vs_oracle(uk, prec):
         assert l2 < tol and mx < tol, (name, l2, mx)
 
 
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("uk", [11, 13])
+def test_vjp_multi_tile_waves(uk, prec, monkeypatch):
+    # 8200 x 4100 (neither a multiple of the tiles) split into 2 waves per CU: on a 256-CU device each
+    # wave of the single-pass adjoint then walks 3 target tiles x 3 source tiles, so the per-wave
+    # loops, the partial read-modify-write and the ragged last tiles all run
+    monkeypatch.setenv("STKFMM_VJP_WAVES_PER_CU", "2")
+    say(f"vjp multi-tile uker={uk} prec={prec} ns=4100 nt=8200 waves/CU=2")
+    rng = np.random.default_rng(700 + uk)
+    ns, nt = 4100, 8200
+    src = rng.random((ns, 3))
+    trg = np.concatenate([src[:37], rng.random((nt - 37, 3))])
+    v = source_values(uk, ns, rng)
+    w = rng.normal(size=(nt, O.trg_dim(uk)))
+    ref = O.direct_vjp(uk, src, v, trg, w)
+    got = S.vjp_device(uk, to_dev(src), to_dev(v), to_dev(trg), to_dev(w), precision=prec)
+    tol = 1e-11 if prec == 0 else 1e-5
+    for g, r, name in zip(got, ref, ("g_src_value", "g_src_coord", "g_trg_coord")):
+        l2, mx = O.max_error(host(g), r)
+        assert l2 < tol and mx < tol, (name, l2, mx)
+
+
 def test_autograd_function_matches_vjp():
     import torch
     from stkfmm_amd.autograd import p2p
