@@ -42,26 +42,22 @@ namespace stkp2p {
 // FP64: v_rsq_f64 seed (measured on gfx950: 24.2 correct bits over r2 in [1e-30, 1e3]) refined by
 // STKP2P_FP64_NEWTON Newton steps y <- y + y (1 - r2 y^2)/2 (one step: 47.8 bits, max relative
 // error 4e-15; two: 52.4 bits), written as e = 0.5 - (r2/2) y^2, y = y + y e.
-// The mask acts on the seed: r2 = +0 gives v_rsq_f64 = +inf = 0x7ff0000000000000, whose low word is
-// already 0, so clearing the high word makes the seed +0 and every Newton step keeps it 0 -- one
-// 32-bit select instead of a 64-bit select after the refinement.  (r2 is a sum of squares: +0 or
-// positive, never -0 or negative.)
-// MASK = false drops the select, for pairs known to have r2 > 0 (the result is then the same).
+// MASK: 0 unless r2 > 0 -- a select after the refinement (a NaN r2 must give 0 like the reference's
+// mask, and a Newton step turns any seed into NaN when r2 is NaN).  MASK = false drops the select,
+// for pairs known to have r2 > 0 (the result is then the same).
 template <bool MASK = true>
 __device__ __forceinline__ double rsqrt_f64(double r2) {
     double y = __builtin_amdgcn_rsq(r2);
-    if constexpr (MASK) {
-        unsigned long long bits = __builtin_bit_cast(unsigned long long, y);
-        const unsigned hi = (r2 > 0.0) ? (unsigned)(bits >> 32) : 0u;
-        y = __builtin_bit_cast(double, ((unsigned long long)hi << 32) | (bits & 0xffffffffull));
-    }
     const double hr = 0.5 * r2;
 #pragma unroll
     for (int i = 0; i < STKP2P_FP64_NEWTON; ++i) {
         const double e = __builtin_fma(-hr, y * y, 0.5);
         y = __builtin_fma(y, e, y);
     }
-    return y;
+    if constexpr (MASK)
+        return (r2 > 0.0) ? y : 0.0;
+    else
+        return y;
 }
 
 // FP32: v_rsq_f32 seed (measured on gfx950: 23.2 correct bits, i.e. within 1.1e-7 relative) and
@@ -479,15 +475,21 @@ struct UkRpyU {
 
 // rpy_ulapu, RPYKernel.hpp:58-89; source [fx, fy, fz, a].  With c_i = f_i/r^3 - 3 (f.r) r_i/r^5:
 //   u_i = f_i/r + r_i (f.r)/r^3 + beta c_i,  lap u_i = 2 c_i  (beta = one_over_three a^2)
-// evaluated as u_i = f_i (1/r + beta/r^3) + r_i (g3 - 3 beta g5), lap u_i/2 = f_i/r^3 - 3 g5 r_i
-// (g3 = (f.r)/r^3, g5 = (f.r)/r^5); post() doubles lap u.  Staged as UkRpyU.
+// evaluated as u_i = f_i (1/r + beta/r^3) + r_i (g3 - 3 beta g5) and lap u_i = -6 (f'_i/r^3 + g5 r_i)
+// with f' = -f/3 staged (g3 = (f.r)/r^3, g5 = (f.r)/r^5; post() applies the -6).  Staged as UkRpyU,
+// then -f/3 (the three extra values fill the 16-byte LDS row that holds the first five).
 struct UkRpyULapU {
-    static constexpr int KS = 4, KS1 = 4, KL = 5, KT = 6;
+    static constexpr int KS = 4, KS1 = 4, KL = 8, KT = 6;
     static constexpr bool UNMASKED_EVAL = true;
     static constexpr unsigned SRC_SCALE_MASK = 0x8u;
     __host__ __device__ static constexpr int deg(int k) { return k < 3 ? 1 : 3; }
-    __host__ __device__ static constexpr double post(int k) { return k < 3 ? 1.0 : 2.0; }
-    __device__ static void stage(const double *in, double *o) { UkRpyU::stage(in, o); }
+    __host__ __device__ static constexpr double post(int k) { return k < 3 ? 1.0 : -6.0; }
+    __device__ static void stage(const double *in, double *o) {
+        UkRpyU::stage(in, o);
+        o[5] = in[0] * (-1.0 / 3.0);
+        o[6] = in[1] * (-1.0 / 3.0);
+        o[7] = in[2] * (-1.0 / 3.0);
+    }
     template <class R, bool M = true>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
@@ -499,19 +501,18 @@ struct UkRpyULapU {
         R g5 = g3 * rinv2;
         R A = f[3] * rinv3 + rinv;
         R B = f[4] * g5 + g3;
-        R m = R(-3.0) * g5;
         u[0] += f[0] * A;
         u[0] += dx * B;
         u[1] += f[1] * A;
         u[1] += dy * B;
         u[2] += f[2] * A;
         u[2] += dz * B;
-        u[3] += f[0] * rinv3;
-        u[3] += dx * m;
-        u[4] += f[1] * rinv3;
-        u[4] += dy * m;
-        u[5] += f[2] * rinv3;
-        u[5] += dz * m;
+        u[3] += f[5] * rinv3;
+        u[3] += dx * g5;
+        u[4] += f[6] * rinv3;
+        u[4] += dy * g5;
+        u[5] += f[7] * rinv3;
+        u[5] += dz * g5;
     }
 };
 

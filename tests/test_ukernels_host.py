@@ -130,3 +130,11 @@ def test_fp32_close_range_finite(uk):
         # 1e-3 of its terms); a literal power that overflowed or lost range would miss by far more
         scale = np.abs(b).max()
         assert np.abs(a - b).max() <= 2e-4 * scale, (r, np.abs(a - b).max() / scale)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_nan_displacement_is_masked(prec):
+    # the reference's mask r2 > 0 is false for NaN, so 1/r is 0: the Laplace potential of such a pair
+    # is 0 (kernels with r_i factors give NaN, as the reference does)
+    for r in ([np.nan, 0.1, 0.2], [0.3, np.nan, np.nan]):
+        assert np.array_equal(host_pair(0, prec, np.array(r), np.array([1.5])), [0.0])
