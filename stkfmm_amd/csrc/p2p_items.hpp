@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemA
                     if constexpr (CLEAR)
                         near = box.near(x, y, z);
                     double vst[KL];
-                    K::stage(vin, vst);
+                    stage_for<K, PREC>(vin, vst);
                     if constexpr (HILO) {
                         const float hx = (float)x, hy = (float)y, hz = (float)z;
                         s_vf[lane][0] = hx;

@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(256) stage_records_kernel(StageArgs a) {
             vin[k] *= lam;
     }
     double vst[KL];
-    K::stage(vin, vst);
+    stage_for<K, PREC>(vin, vst);
     double *rec = a.rec + ((long long)copy * a.n_src + s) * L::WORDS;
     rec[0] = a.sc.p[s * a.sc.s] * lam;
     rec[1] = a.sc.p[s * a.sc.s + a.sc.fs] * lam;

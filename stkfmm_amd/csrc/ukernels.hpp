@@ -14,7 +14,8 @@
 //   stage(in, out)  per-source precomputation in FP64.  Sub-expressions the reference evaluates
 //         from source values alone are hoisted here (e.g. sxy + syx, a*a/3), each with the same
 //         operands and operation, so hoisting does not change a single rounding; the rewritten hot
-//         kernels also stage source-only terms of their rewrite (stated next to their code),
+//         kernels also stage source-only terms of their rewrite (stated next to their code); a
+//         kernel whose FP64 and FP32 forms differ declares STAGE_BY_PRECISION and stage_p<PREC>,
 //   eval<R, M>(u, dx, dy, dz, f)  u += K(r, f) / post(k) without the scale factor, r = x_trg -
 //         x_src; M = false for pairs known to have r2 > 0 (no coincidence mask),
 //   post(k) constant factor applied to accumulated output k once per work item (so that per-pair
@@ -102,6 +103,21 @@ template <class K, class = void> struct HasUnmaskedEval { static constexpr bool 
 template <class K> struct HasUnmaskedEval<K, decltype((void)K::UNMASKED_EVAL)> {
     static constexpr bool value = K::UNMASKED_EVAL;
 };
+
+// stage() of a micro-kernel for the precision its pairs run in: micro-kernels that declare
+// STAGE_BY_PRECISION stage their values differently for FP64 (PREC 0) and FP32 (PREC 1) pairs
+// (stage_p<PREC>); the others have one stage().
+template <class K, class = void> struct StagesByPrecision { static constexpr bool value = false; };
+template <class K> struct StagesByPrecision<K, decltype((void)K::STAGE_BY_PRECISION)> {
+    static constexpr bool value = K::STAGE_BY_PRECISION;
+};
+template <class K, int PREC>
+__host__ __device__ inline void stage_for(const double *in, double *o) {
+    if constexpr (StagesByPrecision<K>::value)
+        K::template stage_p<PREC>(in, o);
+    else
+        K::stage(in, o);
+}
 
 // reference constant one_over_three = 0.3333333333333 (RPYKernel.hpp:30,66)
 #define STKP2P_RPY_ONE_OVER_THREE 0.3333333333333
@@ -326,36 +342,53 @@ struct UkStokesPVel {
 
 // stokes_doublepvel, StokesDoubleLayerKernel.hpp:31-56; source D row-major [Dxx..Dzz].
 //   Q = r.D.r,  p = 2 (tr D/r^3 - 3 Q/r^5),  u_i = -3 r_i Q/r^5.
-// Staged: -3 Dxx, -3 Dyy, -3 Dzz, -3 (Dxy+Dyx), -3 (Dxz+Dzx), -3 (Dyz+Dzy), Dxx+Dyy+Dzz (the
-// reference's source-only sub-expressions, :46-48,51, with its factor -3 of :49 folded in).
-// Rewrite: Q' = -3 Q by Horner in dx, dy, dz (reusing dz^2 of r2), t = Q'/r^2, q = t/r^3 (= -3 Q/r^5
-// with no power above 1/r^3), p/2 = (t + tr D)/r^3, u_i = r_i q; post() doubles p.
+// The reference's source-only sub-expressions (:46-48,51) with its factor -3 (:49) folded in: g0..g2 =
+// -3 Dxx, -3 Dyy, -3 Dzz, g3..g5 = -3 (Dxy+Dyx), -3 (Dxz+Dzx), -3 (Dyz+Dzy), and tr D.
+// Rewrite: t = -3 Q/r^2 (no power above 1/r^3), p/2 = (t + tr D)/r^3, u_i = r_i t/r^3; post() doubles
+// p.  -3 Q by precision:
+//   FP32: Horner in dx, dy, dz reusing dz^2 of r^2; staged g0..g5, tr D.
+//   FP64: with z^2 = r^2 - x^2 - y^2, -3 Q = g2 r^2 + rest, rest = x ((g0 - g2) x + g3 y + g4 z) +
+//         y ((g1 - g2) y + g5 z), so t = g2 + rest/r^2 -- one operation fewer; staged g0 - g2,
+//         g1 - g2, g2, g3, g4, g5, tr D.  (In FP32 the split adds rounding of size |g2| to t, measured
+//         up to 3x the Horner form's error on clustered U-lists, so FP32 keeps the Horner form.)
 struct UkStokesDoublePVel {
     static constexpr int KS = 9, KS1 = 9, KL = 7, KT = 4;
-    static constexpr bool UNMASKED_EVAL = true;
+    static constexpr bool UNMASKED_EVAL = true, STAGE_BY_PRECISION = true;
     static constexpr unsigned SRC_SCALE_MASK = 0x1FFu;
     __host__ __device__ static constexpr int deg(int k) { return k == 0 ? 2 : 1; }
     __host__ __device__ static constexpr double post(int k) { return k == 0 ? 2.0 : 1.0; }
-    __device__ static void stage(const double *in, double *o) {
-        o[0] = -3.0 * in[0];
-        o[1] = -3.0 * in[4];
-        o[2] = -3.0 * in[8];
+    template <int PREC>
+    __host__ __device__ static void stage_p(const double *in, double *o) {
+        const double g0 = -3.0 * in[0], g1 = -3.0 * in[4], g2 = -3.0 * in[8];
+        o[0] = (PREC == 0) ? g0 - g2 : g0;
+        o[1] = (PREC == 0) ? g1 - g2 : g1;
+        o[2] = g2;
         o[3] = -3.0 * (in[1] + in[3]);
         o[4] = -3.0 * (in[2] + in[6]);
         o[5] = -3.0 * (in[5] + in[7]);
         o[6] = in[0] + in[4] + in[8];
     }
+    // -3 r.D.r / r^2 from the first six staged values (the layout of stage_p for R's precision), z2 = dz^2
+    template <class R>
+    __device__ static R t_of(R dx, R dy, R dz, R z2, R rinv2, const R *g) {
+        if constexpr (sizeof(R) == 8) {
+            const R a1 = g[0] * dx + g[3] * dy + g[4] * dz;
+            const R a2 = g[1] * dy + g[5] * dz;
+            return (dx * a1 + dy * a2) * rinv2 + g[2];
+        } else {
+            const R a1 = g[0] * dx + g[3] * dy + g[4] * dz;
+            const R a2 = g[1] * dy + g[5] * dz;
+            return (dx * a1 + (dy * a2 + g[2] * z2)) * rinv2;
+        }
+    }
     template <class R, bool M = true>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *g) {
-        R z2 = dz * dz;
+        const R z2 = dz * dz;
         R r2 = dx * dx + (dy * dy + z2);
         const auto rs = pair_rsqrt<M>(r2);
         R rinv = rs.s, rinv2 = rs.s2;
         R rinv3 = rinv2 * rinv;
-        R a1 = g[0] * dx + g[3] * dy + g[4] * dz;
-        R a2 = g[1] * dy + g[5] * dz;
-        R qq = dx * a1 + (dy * a2 + g[2] * z2);
-        R t = qq * rinv2;
+        R t = t_of(dx, dy, dz, z2, rinv2, g);
         R q = t * rinv3;
         u[0] += (t + g[6]) * rinv3;
         u[1] += dx * q;
@@ -370,37 +403,58 @@ struct UkStokesDoublePVel {
 //   p/2 = (cc + tr D)/r^3 + Q'/r^5,   u_i = f_i/r + r_i ((cc - TrD)/r^3 + Q'/r^5);  post() doubles p.
 // Rewrite (one factor 1/r^3 pulled out of both layers):
 //   t = (cc - TrD) + Q'/r^2,   u_i = f_i/r + r_i (t/r^3),   p/2 = (t + (TrD + tr D))/r^3,
-// with cc - TrD formed as one fused multiply-add chain started from the staged -TrD, and
-// Q' = dx (g0 dx + g3 dy + g4 dz) + dy (g1 dy + g5 dz) + g2 dz^2 reusing dz^2 of r2.
-// Staged: fx, fy, fz, -TrD, then UkStokesDoublePVel's first six values g0..g5 (the symmetrised,
-// -3-scaled D), then TrD + tr D.  26 operations per pair besides the displacement and the reciprocal
-// square root, against 30 for the two layers' expressions side by side.
+// with cc - TrD formed as one fused multiply-add chain started from the staged constant.  Q'/r^2 as
+// in UkStokesDoublePVel, by precision:
+//   FP32: Q' by Horner reusing dz^2; staged fx, fy, fz, -TrD, g0..g5, TrD + tr D.
+//   FP64: Q'/r^2 = g2 + rest/r^2 with g2 folded into the chain's constant; staged fx, fy, fz,
+//         -TrD + g2, g0 - g2, g1 - g2, g3, g4, g5, TrD + tr D (the last slot unused).
+// FP32: 26 operations per pair besides the displacement and the reciprocal square root; FP64: 25;
+// the two layers' expressions side by side: 30.
 struct UkStokesPVelFused {
     static constexpr int KS = 13, KS1 = 4, KL = 11, KT = 4;
-    static constexpr bool UNMASKED_EVAL = true;
+    static constexpr bool UNMASKED_EVAL = true, STAGE_BY_PRECISION = true;
     static constexpr unsigned SRC_SCALE_MASK = 0x1FF8u;
     __host__ __device__ static constexpr int deg(int k) { return k == 0 ? 2 : 1; }
     __host__ __device__ static constexpr double post(int k) { return k == 0 ? 2.0 : 1.0; }
-    __device__ static void stage(const double *in, double *o) {
+    template <int PREC>
+    __host__ __device__ static void stage_p(const double *in, double *o) {
+        double d[7];
+        UkStokesDoublePVel::stage_p<PREC>(in + 4, d);
         o[0] = in[0]; o[1] = in[1]; o[2] = in[2];
-        o[3] = -in[3];
-        UkStokesDoublePVel::stage(in + 4, o + 4);
-        o[10] = in[3] + o[10];
+        if constexpr (PREC == 0) {
+            o[3] = -in[3] + d[2];
+            o[4] = d[0]; o[5] = d[1]; o[6] = d[3]; o[7] = d[4]; o[8] = d[5];
+            o[9] = in[3] + d[6];
+            o[10] = 0.0;
+        } else {
+            o[3] = -in[3];
+            for (int k = 0; k < 6; ++k)
+                o[4 + k] = d[k];
+            o[10] = in[3] + d[6];
+        }
     }
     template <class R, bool M = true>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
-        R z2 = dz * dz;
+        const R z2 = dz * dz;
         R r2 = dx * dx + (dy * dy + z2);
         const auto rs = pair_rsqrt<M>(r2);
         R rinv = rs.s, rinv2 = rs.s2;
         R rinv3 = rinv2 * rinv;
         R cm = f[0] * dx + (f[1] * dy + (f[2] * dz + f[3]));
-        R a1 = f[4] * dx + f[7] * dy + f[8] * dz;
-        R a2 = f[5] * dy + f[9] * dz;
-        R qq = dx * a1 + (dy * a2 + f[6] * z2);
-        R t = qq * rinv2 + cm;
+        R t, G;
+        if constexpr (sizeof(R) == 8) {
+            const R a1 = f[4] * dx + f[6] * dy + f[7] * dz;
+            const R a2 = f[5] * dy + f[8] * dz;
+            t = (dx * a1 + dy * a2) * rinv2 + cm;
+            G = f[9];
+        } else {
+            const R a1 = f[4] * dx + f[7] * dy + f[8] * dz;
+            const R a2 = f[5] * dy + f[9] * dz;
+            t = (dx * a1 + (dy * a2 + f[6] * z2)) * rinv2 + cm;
+            G = f[10];
+        }
         R b = t * rinv3;
-        u[0] += (t + f[10]) * rinv3;
+        u[0] += (t + G) * rinv3;
         u[1] += f[0] * rinv;
         u[1] += dx * b;
         u[2] += f[1] * rinv;

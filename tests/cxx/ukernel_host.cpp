@@ -13,7 +13,7 @@ void pair(const double *r, const double *fin, double scale, double *u) {
     double vin[K::KS];
     for (int k = 0; k < K::KS; ++k)
         vin[k] = fin[k];
-    K::stage(vin, vst);
+    stage_for<K, sizeof(R) == 8 ? 0 : 1>(vin, vst);
     R f[K::KL];
     for (int k = 0; k < K::KL; ++k)
         f[k] = (R)vst[k];
