@@ -58,6 +58,12 @@ namespace stkp2p {
 #define STKP2P_ITEM_ATTR
 #endif
 
+// unroll factor of the mixed-mode source loop: MIXED_UNROLL of the micro-kernel, else 2
+template <class K, class = void> struct MixedUnroll { static constexpr int value = 2; };
+template <class K> struct MixedUnroll<K, decltype((void)K::MIXED_UNROLL)> {
+    static constexpr int value = K::MIXED_UNROLL;
+};
+
 template <int UK, int PREC, int T>
 __global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemArgs a) {
     using K = typename UkerOf<UK>::type;
@@ -204,7 +210,10 @@ __global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemA
 #pragma unroll
                                 for (int k = 0; k < KT; ++k)
                                     part[t][k] = 0.0f;
-#pragma unroll 2
+                            // sources per iteration: MixedUnroll<K> (measured on MI355X, T = 2: the
+                            // fused PVel U-list 8.37 ms at 1, 8.62 at 2, 8.91 at 4; RPY 1 % faster
+                            // at 2 than at 1)
+#pragma unroll MixedUnroll<K>::value
                             for (int j = j0 + j_first; j < j1; j += j_step) {
                                 // coordinates first: LDS reads return in issue order, and the
                                 // displacement is needed before the values

@@ -413,6 +413,7 @@ struct UkStokesDoublePVel {
 struct UkStokesPVelFused {
     static constexpr int KS = 13, KS1 = 4, KL = 11, KT = 4;
     static constexpr bool UNMASKED_EVAL = true, STAGE_BY_PRECISION = true;
+    static constexpr int MIXED_UNROLL = 1; // item kernel's mixed-mode source loop (measured fastest)
     static constexpr unsigned SRC_SCALE_MASK = 0x1FF8u;
     __host__ __device__ static constexpr int deg(int k) { return k == 0 ? 2 : 1; }
     __host__ __device__ static constexpr double post(int k) { return k == 0 ? 2.0 : 1.0; }
