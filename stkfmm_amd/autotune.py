@@ -26,7 +26,7 @@ DEFAULTS_PATH = os.path.join(HERE, "tuned_defaults.json")
 CANDIDATES = ([dict(targets_per_lane=t, max_src_per_item=m, persistent=p, broadcast=0)
                for t, m, p in itertools.product((1, 2, 4), (1024, 2048, 4096), (1, 2))]
               + [dict(targets_per_lane=t, max_src_per_item=m, persistent=p, broadcast=1)
-                 for t, m, p in itertools.product((1, 2), (1024, 2048), (1, 2))]
+                 for t, m, p in itertools.product((1, 2, 4), (1024, 2048), (1, 2))]
               + [dict(targets_per_lane=t, max_src_per_item=m, persistent=1, queues=8)
                  for t, m in itertools.product((1, 2), (1024, 2048))])
 
