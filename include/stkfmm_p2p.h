@@ -355,8 +355,9 @@ int stkfmm_wall_eval_device(unsigned kernel, int precision, int nSL, const doubl
 /* ---- adjoints ----------------------------------------------------------------------------- */
 
 /* Gradients of L = sum_t w_t . u_t for u = direct-sum P2P (same conventions as
- * stkfmm_p2p_eval_device), for STKFMM_UKER_STOKES_PVEL, STKFMM_UKER_STOKES_DOUBLEPVEL,
- * STKFMM_UKER_STOKES_PVEL_FUSED and STKFMM_UKER_RPY_ULAPU.  No reference counterpart (the reference
+ * stkfmm_p2p_eval_device), for STKFMM_UKER_LAPLACE_P, _LAPLACE_PGRAD, _STOKES_VEL, _STOKES_REGVEL
+ * (including d/d eps), _RPY_U, _RPY_ULAPU, _STK_ULAPU, _STOKES_PVEL, _STOKES_DOUBLEPVEL and
+ * _STOKES_PVEL_FUSED.  No reference counterpart (the reference
  * has no derivatives); callers: the torch.autograd.Function in the Python binding.
  * Device AoS arrays.  Outputs are OVERWRITTEN: g_src_value [nSrc*ks], g_src_coord [nSrc*3],
  * g_trg_coord [nTrg*3]; any of them may be NULL to skip it.  For the fused kernel src_value holds

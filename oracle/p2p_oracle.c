@@ -1050,9 +1050,90 @@ static void vjp_stokes_pvel_fused(const double *r, const double *f, const double
     gr[2] += gr2[2];
 }
 
+/* laplace_p: K = q/r */
+static void vjp_laplace_p(const double *r, const double *f, const double *w, double *gr, double *gf) {
+    double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    double rinv = rsqrt_masked(r2);
+    double c = -w[0] * f[0] * rinv * rinv * rinv;
+    for (int i = 0; i < 3; i++)
+        gr[i] = c * r[i];
+    gf[0] = w[0] * rinv;
+}
+
+/* laplace_pgrad: K = [q/r, -q r/r^3] */
+static void vjp_laplace_pgrad(const double *r, const double *f, const double *w, double *gr, double *gf) {
+    double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    double rinv = rsqrt_masked(r2);
+    double rinv3 = rinv * rinv * rinv, rinv5 = rinv3 * rinv * rinv;
+    double q = f[0];
+    double wg = w[1] * r[0] + w[2] * r[1] + w[3] * r[2];
+    for (int i = 0; i < 3; i++)
+        gr[i] = -w[0] * q * r[i] * rinv3 - q * w[1 + i] * rinv3 + 3.0 * q * wg * r[i] * rinv5;
+    gf[0] = w[0] * rinv - wg * rinv3;
+}
+
+/* stokes_vel (Stokeslet): K = f/r + r (f.r)/r^3 */
+static void vjp_stokes_vel(const double *r, const double *f, const double *w, double *gr, double *gf) {
+    double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    double rinv = rsqrt_masked(r2);
+    double rinv3 = rinv * rinv * rinv, rinv5 = rinv3 * rinv * rinv;
+    double a = f[0] * r[0] + f[1] * r[1] + f[2] * r[2];
+    double b = w[0] * r[0] + w[1] * r[1] + w[2] * r[2];
+    double wf = w[0] * f[0] + w[1] * f[1] + w[2] * f[2];
+    for (int i = 0; i < 3; i++) {
+        gr[i] = -wf * r[i] * rinv3 + (w[i] * a + b * f[i]) * rinv3 - 3.0 * a * b * r[i] * rinv5;
+        gf[i] = w[i] * rinv + r[i] * b * rinv3;
+    }
+}
+
+/* stokes_regvel: rho^2 = r^2 + eps^2, K = ((rho^2 + eps^2) f + r (f.r))/rho^3; source [f(3), eps] */
+static void vjp_stokes_regvel(const double *r, const double *f, const double *w, double *gr, double *gf) {
+    double eps = f[3];
+    double rr = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    double rho2 = rr + eps * eps;
+    double s = rsqrt_masked(rho2);
+    double s3 = s * s * s, s5 = s3 * s * s;
+    double h = rr + 2.0 * eps * eps; /* = rho^2 + eps^2 */
+    double a = f[0] * r[0] + f[1] * r[1] + f[2] * r[2];
+    double b = w[0] * r[0] + w[1] * r[1] + w[2] * r[2];
+    double wf = w[0] * f[0] + w[1] * f[1] + w[2] * f[2];
+    for (int i = 0; i < 3; i++) {
+        gr[i] = wf * r[i] * (2.0 * s3 - 3.0 * h * s5) + (w[i] * a + b * f[i]) * s3 - 3.0 * a * b * r[i] * s5;
+        gf[i] = w[i] * h * s3 + r[i] * b * s3;
+    }
+    gf[3] = wf * eps * (4.0 * s3 - 3.0 * h * s5) - 3.0 * a * b * eps * s5;
+}
+
+/* rpy_u: the velocity of rpy_ulapu (cotangent of the Laplacian zero) */
+static void vjp_rpy_u(const double *r, const double *f, const double *w, double *gr, double *gf) {
+    double w6[6] = {w[0], w[1], w[2], 0.0, 0.0, 0.0};
+    vjp_rpy_ulapu(r, f, w6, gr, gf);
+}
+
+/* stk_ulapu: rpy_ulapu with b = 0 (three source values) */
+static void vjp_stk_ulapu(const double *r, const double *f, const double *w, double *gr, double *gf) {
+    double f4[4] = {f[0], f[1], f[2], 0.0}, gf4[4];
+    vjp_rpy_ulapu(r, f4, w, gr, gf4);
+    gf[0] = gf4[0];
+    gf[1] = gf4[1];
+    gf[2] = gf4[2];
+}
+
 typedef void (*vjp_fn)(const double *, const double *, const double *, double *, double *);
 static vjp_fn vjp_lookup(int uk) {
     switch (uk) {
+    case UK_LAPLACE_P:
+        return vjp_laplace_p;
+    case UK_LAPLACE_PGRAD:
+        return vjp_laplace_pgrad;
+    case UK_STOKES_VEL:
+        return vjp_stokes_vel;
+    case UK_STOKES_REGVEL:
+        return vjp_stokes_regvel;
+    case UK_RPY_U:
+        return vjp_rpy_u;
+    case UK_STK_ULAPU:
+        return vjp_stk_ulapu;
     case UK_STOKES_PVEL:
         return vjp_stokes_pvel;
     case UK_STOKES_DOUBLEPVEL:

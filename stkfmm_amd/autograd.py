@@ -3,14 +3,16 @@
 u = P2PFunction.apply(uker, precision, src_coord, src_value, trg_coord) returns the [n_trg, kt]
 direct sum (scale factor applied, coincident pairs masked).  backward() calls
 stkfmm_p2p_vjp_device, which returns dL/d(src_value), dL/d(src_coord) and dL/d(trg_coord) for the
-incoming cotangent.  Supported micro-kernels: STOKES_PVEL, STOKES_DOUBLEPVEL, STOKES_PVEL_FUSED,
-RPY_ULAPU.  The reference has no derivatives; this is new API.
+incoming cotangent.  Supported micro-kernels: LAPLACE_P, LAPLACE_PGRAD, STOKES_VEL (Stokeslet),
+STOKES_REGVEL (gradient with respect to eps included), RPY_U, RPY_ULAPU, STK_ULAPU, STOKES_PVEL,
+STOKES_DOUBLEPVEL, STOKES_PVEL_FUSED.  The reference has no derivatives; this is new API.
 """
 import torch
 
 from . import PREC, UKER, uker_device, uker_dims, vjp_device
 
-_ADJOINT_UKERS = (UKER.STOKES_PVEL, UKER.STOKES_DOUBLEPVEL, UKER.STOKES_PVEL_FUSED, UKER.RPY_ULAPU)
+_ADJOINT_UKERS = (UKER.LAPLACE_P, UKER.LAPLACE_PGRAD, UKER.STOKES_VEL, UKER.STOKES_REGVEL, UKER.RPY_U,
+                  UKER.RPY_ULAPU, UKER.STK_ULAPU, UKER.STOKES_PVEL, UKER.STOKES_DOUBLEPVEL, UKER.STOKES_PVEL_FUSED)
 
 
 class P2PFunction(torch.autograd.Function):

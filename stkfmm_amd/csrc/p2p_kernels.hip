@@ -204,12 +204,21 @@ static hipError_t launch_vjp_p(const VjpArgs &a, const VjpLayout &l, double *par
     return (l.T == 1) ? launch_vjp_t<UK, PREC, 1>(a, l, partial, s) : launch_vjp_t<UK, PREC, 2>(a, l, partial, s);
 }
 
-bool uker_has_vjp(int uk) { return uk == 7 || uk == 8 || uk == 11 || uk == 13; }
+bool uker_has_vjp(int uk) {
+    return uk == 0 || uk == 2 || uk == 6 || uk == 7 || uk == 8 || uk == 9 || uk == 10 || uk == 11 || uk == 12 ||
+           uk == 13;
+}
 
 hipError_t launch_vjp(int uk, int prec, const VjpArgs &a, const VjpLayout &l, double *partial, hipStream_t s) {
     if (l.T != 1 && l.T != 2)
         return hipErrorInvalidValue;
     switch (uk) {
+    case 0: return prec ? launch_vjp_p<0, 1>(a, l, partial, s) : launch_vjp_p<0, 0>(a, l, partial, s);
+    case 2: return prec ? launch_vjp_p<2, 1>(a, l, partial, s) : launch_vjp_p<2, 0>(a, l, partial, s);
+    case 6: return prec ? launch_vjp_p<6, 1>(a, l, partial, s) : launch_vjp_p<6, 0>(a, l, partial, s);
+    case 9: return prec ? launch_vjp_p<9, 1>(a, l, partial, s) : launch_vjp_p<9, 0>(a, l, partial, s);
+    case 10: return prec ? launch_vjp_p<10, 1>(a, l, partial, s) : launch_vjp_p<10, 0>(a, l, partial, s);
+    case 12: return prec ? launch_vjp_p<12, 1>(a, l, partial, s) : launch_vjp_p<12, 0>(a, l, partial, s);
     case 7: return prec ? launch_vjp_p<7, 1>(a, l, partial, s) : launch_vjp_p<7, 0>(a, l, partial, s);
     case 8: return prec ? launch_vjp_p<8, 1>(a, l, partial, s) : launch_vjp_p<8, 0>(a, l, partial, s);
     case 11: return prec ? launch_vjp_p<11, 1>(a, l, partial, s) : launch_vjp_p<11, 0>(a, l, partial, s);

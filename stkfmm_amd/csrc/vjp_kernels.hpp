@@ -126,7 +126,125 @@ struct VjpStokesPVelFused {
     }
 };
 
+// laplace_p: K = q/r
+struct VjpLaplaceP {
+    static constexpr int KS = 1, KT = 1;
+    template <class R>
+    __device__ static void vjp(R dx, R dy, R dz, const R *f, const R *w, R *gr, R *gf) {
+        R r2 = dx * dx + dy * dy + dz * dz;
+        R rinv = rsqrt_masked(r2);
+        R c = -w[0] * f[0] * (rinv * rinv * rinv);
+        gr[0] += c * dx;
+        gr[1] += c * dy;
+        gr[2] += c * dz;
+        gf[0] += w[0] * rinv;
+    }
+};
+
+// laplace_pgrad: K = [q/r, -q r/r^3]
+struct VjpLaplacePGrad {
+    static constexpr int KS = 1, KT = 4;
+    template <class R>
+    __device__ static void vjp(R dx, R dy, R dz, const R *f, const R *w, R *gr, R *gf) {
+        const R r[3] = {dx, dy, dz};
+        R r2 = dx * dx + dy * dy + dz * dz;
+        R rinv = rsqrt_masked(r2);
+        R rinv3 = rinv * (rinv * rinv), rinv5 = rinv3 * (rinv * rinv);
+        R wg = w[1] * dx + w[2] * dy + w[3] * dz;
+        R q = f[0];
+        R cr = q * (R(3) * wg * rinv5 - w[0] * rinv3);
+        R cw = -q * rinv3;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            gr[i] += cr * r[i] + cw * w[1 + i];
+        gf[0] += w[0] * rinv - wg * rinv3;
+    }
+};
+
+// stokes_vel (Stokeslet): K = f/r + r (f.r)/r^3
+struct VjpStokesVel {
+    static constexpr int KS = 3, KT = 3;
+    template <class R>
+    __device__ static void vjp(R dx, R dy, R dz, const R *f, const R *w, R *gr, R *gf) {
+        const R r[3] = {dx, dy, dz};
+        R r2 = dx * dx + dy * dy + dz * dz;
+        R rinv = rsqrt_masked(r2);
+        R rinv3 = rinv * (rinv * rinv), rinv5 = rinv3 * (rinv * rinv);
+        R a = f[0] * dx + f[1] * dy + f[2] * dz;
+        R b = w[0] * dx + w[1] * dy + w[2] * dz;
+        R wf = w[0] * f[0] + w[1] * f[1] + w[2] * f[2];
+        R ar3 = a * rinv3, br3 = b * rinv3;
+        R cr = wf * rinv3 + R(3) * a * b * rinv5;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            gr[i] += ar3 * w[i] + br3 * f[i] - cr * r[i];
+            gf[i] += w[i] * rinv + br3 * r[i];
+        }
+    }
+};
+
+// stokes_regvel: rho^2 = r^2 + eps^2, K = ((rho^2 + eps^2) f + r (f.r))/rho^3; source [f(3), eps]
+struct VjpStokesRegVel {
+    static constexpr int KS = 4, KT = 3;
+    template <class R>
+    __device__ static void vjp(R dx, R dy, R dz, const R *f, const R *w, R *gr, R *gf) {
+        const R r[3] = {dx, dy, dz};
+        const R eps = f[3];
+        R rr = dx * dx + dy * dy + dz * dz;
+        R s = rsqrt_masked(rr + eps * eps);
+        R s3 = s * (s * s), s5 = s3 * (s * s);
+        R h = rr + R(2) * eps * eps;
+        R a = f[0] * dx + f[1] * dy + f[2] * dz;
+        R b = w[0] * dx + w[1] * dy + w[2] * dz;
+        R wf = w[0] * f[0] + w[1] * f[1] + w[2] * f[2];
+        R as3 = a * s3, bs3 = b * s3, hs3 = h * s3;
+        R ab5 = R(3) * a * b * s5;
+        R cr = wf * (R(2) * s3 - R(3) * h * s5) - ab5;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            gr[i] += cr * r[i] + as3 * w[i] + bs3 * f[i];
+            gf[i] += w[i] * hs3 + bs3 * r[i];
+        }
+        gf[3] += eps * (wf * (R(4) * s3 - R(3) * h * s5) - ab5);
+    }
+};
+
+// rpy_u: the velocity of rpy_ulapu (cotangent of the Laplacian zero)
+struct VjpRpyU {
+    static constexpr int KS = 4, KT = 3;
+    template <class R>
+    __device__ static void vjp(R dx, R dy, R dz, const R *f, const R *w, R *gr, R *gf);
+};
+
+// stk_ulapu: rpy_ulapu with b = 0
+struct VjpStkULapU {
+    static constexpr int KS = 3, KT = 6;
+    template <class R>
+    __device__ static void vjp(R dx, R dy, R dz, const R *f, const R *w, R *gr, R *gf);
+};
+
 template <int UK> struct VjpOf { using type = void; };
+template <class R>
+__device__ void VjpRpyU::vjp(R dx, R dy, R dz, const R *f, const R *w, R *gr, R *gf) {
+    const R w6[6] = {w[0], w[1], w[2], R(0), R(0), R(0)};
+    VjpRpyULapU::vjp(dx, dy, dz, f, w6, gr, gf);
+}
+template <class R>
+__device__ void VjpStkULapU::vjp(R dx, R dy, R dz, const R *f, const R *w, R *gr, R *gf) {
+    const R f4[4] = {f[0], f[1], f[2], R(0)};
+    R g4[4] = {gf[0], gf[1], gf[2], R(0)};
+    VjpRpyULapU::vjp(dx, dy, dz, f4, w, gr, g4);
+    gf[0] = g4[0];
+    gf[1] = g4[1];
+    gf[2] = g4[2];
+}
+
+template <> struct VjpOf<0> { using type = VjpLaplaceP; };
+template <> struct VjpOf<2> { using type = VjpLaplacePGrad; };
+template <> struct VjpOf<6> { using type = VjpStokesVel; };
+template <> struct VjpOf<9> { using type = VjpStokesRegVel; };
+template <> struct VjpOf<10> { using type = VjpRpyU; };
+template <> struct VjpOf<12> { using type = VjpStkULapU; };
 template <> struct VjpOf<7> { using type = VjpStokesPVel; };
 template <> struct VjpOf<8> { using type = VjpStokesDoublePVel; };
 template <> struct VjpOf<11> { using type = VjpRpyULapU; };

@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
-@pytest.mark.parametrize("uk", [7, 8, 11, 13])
+@pytest.mark.parametrize("uk", [0, 2, 6, 7, 8, 9, 10, 11, 12, 13])
 def test_vjp_vs_oracle(uk, prec):
     say(f"vjp uker={uk} prec={prec}")
     rng = np.random.default_rng(500 + uk)
@@ -71,6 +71,20 @@ def test_autograd_function_matches_vjp():
     t2 = to_dev(rng.random((3, 3))).requires_grad_(True)
     assert torch.autograd.gradcheck(lambda a, b, c: p2p(S.UKER.RPY_ULAPU, a, b, c), (s2, v2, t2), eps=1e-6,
                                     atol=1e-6, rtol=1e-5)
+
+
+@pytest.mark.parametrize("uk", [0, 2, 6, 9, 10, 12])
+def test_autograd_gradcheck_more_kernels(uk):
+    # torch's finite-difference check through the autograd Function (FP64, tiny problem; for the
+    # regularised Stokeslet the gradient includes d/d eps)
+    import torch
+    from stkfmm_amd.autograd import p2p
+    say(f"gradcheck uker={uk}")
+    rng = np.random.default_rng(620 + uk)
+    s = to_dev(rng.random((5, 3))).requires_grad_(True)
+    v = to_dev(source_values(uk, 5, rng)).requires_grad_(True)
+    t = to_dev(rng.random((4, 3))).requires_grad_(True)
+    assert torch.autograd.gradcheck(lambda a, b, c: p2p(uk, a, b, c), (s, v, t), eps=1e-6, atol=1e-6, rtol=1e-5)
 
 
 @pytest.mark.slow

@@ -194,7 +194,10 @@ def test_translation_invariance(uk):
     np.testing.assert_allclose(O.direct(uk, src + c, v, trg + c), ref, rtol=1e-11, atol=1e-11 * np.abs(ref).max())
 
 
-@pytest.mark.parametrize("uk", [7, 8, 11, 13])
+ADJOINT_UKERS = [0, 2, 6, 7, 8, 9, 10, 11, 12, 13]
+
+
+@pytest.mark.parametrize("uk", ADJOINT_UKERS)
 def test_adjoint_matches_finite_differences(uk):
     rng = np.random.default_rng(80 + uk)
     ns, nt = 5, 4
@@ -220,8 +223,9 @@ def test_adjoint_matches_finite_differences(uk):
         np.testing.assert_allclose(grad, fd, rtol=1e-5, atol=1e-5 * np.abs(fd).max(), err_msg=name)
 
 
-@pytest.mark.parametrize("uk", [7, 8, 11, 13])
+@pytest.mark.parametrize("uk", [u for u in ADJOINT_UKERS if u != 9])
 def test_adjoint_of_coincident_pair_is_zero(uk):
+    # (the regularised Stokeslet is finite at r = 0: its eps > 0 keeps rho > 0)
     rng = np.random.default_rng(90 + uk)
     gr, gf = O.vjp_pair(uk, np.zeros(3), source_values(uk, 1, rng)[0], rng.normal(size=O.trg_dim(uk)))
     assert np.all(gr == 0.0) and np.all(gf == 0.0)
