@@ -34,7 +34,7 @@ PEAK_FP32 = 157.3e12  # MI355X vector FP32 (published, packed), FLOP/s
 # algorithmic FLOPs per pair and the FP64 share in mixed mode (displacement + accumulation)
 FLOPS = {"pvel_fused": (72, 7), "rpy": (56, 9), "stokeslet": (29, 6), "lapgrad": (20, 7)}
 NOTE = ("VALU-bound; peak = MI355X published vector FP64 78.6 / FP32 157.3 TFLOP/s combined by the "
-        "FP64/FP32 split of the algorithmic FLOPs; this build keeps packed FP32 disabled")
+        "FP64/FP32 split of the algorithmic FLOPs; the library is compiled with packed FP32 disabled")
 
 
 def parse():
@@ -371,7 +371,7 @@ def stokeslet_instances(args, torch, S, W):
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print(json.dumps({"error": "this build measures one GPU only: the P2P path runs on a single MI355X "
+        print(json.dumps({"error": "bench.py measures one GPU only: the P2P path runs on a single MI355X "
                                    "(multi-GPU sharding is a design note, not a measured mode)",
                           "n_gpus": args.gpus}))
         sys.exit(2)

@@ -86,7 +86,7 @@ enum stkfmm_status {
     STKFMM_ERR_PPKERNEL = -2,    /* sub-kernel absent (reference: message + return, FMMData.cpp:270-273) */
     STKFMM_ERR_ARG = -3,         /* bad size, pointer or layout */
     STKFMM_ERR_HIP = -4,         /* HIP runtime error */
-    STKFMM_ERR_UNSUPPORTED = -5, /* valid request this build does not implement (e.g. dof != 1) */
+    STKFMM_ERR_UNSUPPORTED = -5, /* valid request the library does not implement (e.g. dof != 1) */
     STKFMM_ERR_INACTIVE = -6     /* kernel not active in the object (reference: exit(1), STKFMM.cpp:92-95) */
 };
 
