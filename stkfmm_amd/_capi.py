@@ -120,6 +120,14 @@ def lib():
             raise ImportError(
                 f"stkfmm_amd: native library {LIB_PATH} is missing; build it with "
                 "`python -c 'import __graft_entry__ as g; g.build()'` or `make -C stkfmm_amd/csrc`")
+        # torch ships its own HIP runtime (soname libamdhip64.so.7, but libtorch_hip asks for the
+        # unversioned libamdhip64.so).  Loaded after this library, torch cannot match the runtime
+        # already in the process and brings a second one; whichever initialises second then finds
+        # no device.  Loading torch first lets this library bind to torch's runtime instead.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _lib = _declare(ctypes.CDLL(LIB_PATH))
     return _lib
 
