@@ -11,6 +11,7 @@ roofline of the dominant kernel (HIP events on its stream) and the CPU baseline 
 vectorised C restatement of the reference P2P on the host cores).
 
 Other workloads (--workload), each printing its own JSON line:
+  pvel-ulist-4m         the default workload's distribution at 4,000,000 points (mixed / FP64 / per-leaf)
   lapgrad-direct        8192 x 8192 Laplace PGrad direct sum (STKFMM::evaluateKernel), FP64
   rpy-ulist             4M log-normal clustered points, leaf cap 2000, RPY, mixed (autotuned) and FP64
   pvel-adjoint          65,536 x 65,536 Stokes PVel forward + adjoint (gradients of sum_t w_t.u_t)
@@ -41,7 +42,8 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="pvel-ulist",
-                    choices=["pvel-ulist", "lapgrad-direct", "rpy-ulist", "pvel-adjoint", "stokeslet-instances"])
+                    choices=["pvel-ulist", "pvel-ulist-4m", "lapgrad-direct", "rpy-ulist", "pvel-adjoint",
+                             "stokeslet-instances"])
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU work budget of the baseline sample")
@@ -399,6 +401,11 @@ def main():
         ulist_bench(args, torch, S, "pvel_fused", S.UKER.STOKES_PVEL_FUSED, W.pvel_points, 1000, W.pvel_values, 4,
                     "Gpair-interactions/s StokesPVel P2P: FP64 vs mixed at ≤1e-6 rel err",
                     "stokes-pvel-ulist-1M-clustered-leafcap1000", ["mixed", "fp64", "per-leaf"])
+    elif args.workload == "pvel-ulist-4m":
+        # the same point distribution at 4M points (the size of the batched-vs-per-leaf target)
+        ulist_bench(args, torch, S, "pvel_fused", S.UKER.STOKES_PVEL_FUSED, lambda: W.pvel_points(4_000_000), 1000,
+                    W.pvel_values, 4, "Gpair-interactions/s StokesPVel P2P: FP64 vs mixed at ≤1e-6 rel err",
+                    "stokes-pvel-ulist-4M-clustered-leafcap1000", ["mixed", "fp64", "per-leaf"])
     elif args.workload == "lapgrad-direct":
         lapgrad_direct(args, torch, S, W)
     elif args.workload == "rpy-ulist":

@@ -186,6 +186,34 @@ def test_workload3_rpy_full_size():
     assert l2 < MIXED_TOL and mx < MIXED_TOL, (l2, mx)
 
 
+@pytest.mark.slow
+def test_workload2_pvel_ulist_4m():
+    """The PVel U-list distribution at 4M points (the batched-vs-per-leaf target size): sampled leaves
+    vs the oracle, mixed and FP64."""
+    say("workload 2 at 4M tree")
+    x = W.pvel_points(4_000_000)
+    tr = S.Tree(x, 1000)
+    xs = x[tr.perm]
+    sl, dl = W.pvel_values(tr.n)
+    d_x, d_sl, d_dl = to_dev(xs), to_dev(sl), to_dev(dl)
+    outs = {}
+    for prec in (1, 0):
+        say(f"workload 2 at 4M prec={prec}")
+        plan = S.Plan.from_tree(tr, 13, prec)
+        out = zeros(tr.n, 4)
+        plan.execute(d_x, d_sl, d_x, out, src_value2=d_dl)
+        outs[prec] = host(out)
+    g = np.random.default_rng(4)
+    sz = tr.leaf_end - tr.leaf_begin
+    sel = np.unique(np.concatenate([g.choice(tr.n_leaves, 10, replace=False), [np.argmax(sz)]]))
+    ref = oracle_leaves(tr, 13, xs, sl, dl, sel)
+    rows = leaf_rows(tr, sel)
+    l2, mx = O.max_error(outs[0][rows], ref[rows])
+    assert l2 < FP64_TOL and mx < FP64_TOL, (l2, mx)
+    l2, mx = O.max_error(outs[1][rows], ref[rows])
+    assert l2 < MIXED_TOL and mx < MIXED_TOL, (l2, mx)
+
+
 def instance_plan(sizes, offsets, prec, uk=6):
     n = int(offsets[-1])
     gb, ge = offsets[:-1], offsets[1:]
