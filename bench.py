@@ -122,6 +122,44 @@ class Timed:
 
 # ---- U-list workloads (headline: PVel) ---------------------------------------------------------------
 
+# micro-kernel -> the reference's Test/SimpleKernel.cpp functions that make it up, as (KERNEL value,
+# layer, first and last source-value column); the fused PVel is the single plus the double layer
+REF_PARTS = {13: ((128, 0, 0, 4), (128, 1, 4, 13)), 11: ((16, 0, 0, 4),)}
+
+
+def reference_sample(O, uker, tree, xs, allv, kt, leaf_order, budget_s, nth):
+    """The reference's own scalar P2P (Test/SimpleKernel.cpp, built into oracle/_ref, OpenMP over
+    targets as in Test/Test.cpp:478) on whole leaves' U-lists, in leaf_order until budget_s seconds
+    of compute are spent.  Returns (timing dict, target rows, reference values of those rows)."""
+    if int(uker) not in REF_PARTS or not O.ref_available():
+        return None, None, None
+    used = O.simplekernel_threads(nth)
+    done, rows, outs, pairs, secs = [], [], [], 0, 0.0
+    for leaf in leaf_order:
+        b, e = int(tree.leaf_begin[leaf]), int(tree.leaf_end[leaf])
+        r0, r1 = int(tree.ulist_ptr[leaf]), int(tree.ulist_ptr[leaf + 1])
+        src = np.concatenate([np.arange(tree.rng_begin[r], tree.rng_end[r]) for r in range(r0, r1)])
+        sc, tc = np.ascontiguousarray(xs[src]), np.ascontiguousarray(xs[b:e])
+        parts = [(K, layer, np.ascontiguousarray(allv[src, c0:c1]), c1 - c0)
+                 for K, layer, c0, c1 in REF_PARTS[int(uker)]]
+        out = np.zeros((e - b, kt))
+        t = time.perf_counter()
+        for K, layer, sv, ks in parts:
+            out += O.simplekernel_direct(K, layer, ks, kt, sc, sv, tc)
+        secs += time.perf_counter() - t
+        pairs += (e - b) * src.size
+        done.append(leaf)
+        rows.append(np.arange(b, e))
+        outs.append(out)
+        if secs >= budget_s:
+            break
+    info = {"value": pairs / secs / 1e9, "unit": "Gpair/s", "cores": used, "kind": "reference",
+            "sample": (f"{len(done)} of {tree.n_leaves} leaves (whole U-lists), {sum(r.size for r in rows)} targets, "
+                       f"{pairs:.3e} pairs, {secs:.1f} s; the reference's scalar Test/SimpleKernel.cpp functions "
+                       f"compiled from its sources (g++ -O2), OpenMP over targets as its runSimpleKernel loop")}
+    return info, np.concatenate(rows), np.concatenate(outs)
+
+
 def ulist_bench(args, torch, S, kind, uker, points, cap, values_fn, kt, metric, workload_name, variants_order):
     from stkfmm_amd.autotune import autotune
 
@@ -182,7 +220,7 @@ def ulist_bench(args, torch, S, kind, uker, points, cap, values_fn, kt, metric, 
         rl["traffic"] = tr.get("hbm_bytes_per_launch")
         rl["traffic_source"] = os.path.relpath(tfile, ROOT)
 
-    accuracy, cpu = None, None
+    accuracy, cpu, ref_cpu = None, None, None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
         nth = threads()
@@ -227,8 +265,18 @@ def ulist_bench(args, torch, S, kind, uker, points, cap, values_fn, kt, metric, 
         for name, res in results.items():
             l2, mx = O.max_error(res.cpu().numpy()[rows], ref[rows])
             accuracy[name] = {"errorL2": l2, "errorMaxRel": mx, "targets_checked": int(rows.size)}
+        # the reference's own CPU code on a sample of the same leaves, timed and used as a second checker
+        ref_cpu, ref_rows, ref_out = reference_sample(O, uker, tree, xs, allv, kt, sel, args.cpu_seconds / 2, nth)
+        if ref_cpu is not None:
+            ref_cpu["cpu_model"] = cpu_model()
+            ref_acc = {}
+            for name, res in results.items():
+                l2, mx = O.max_error(res.cpu().numpy()[ref_rows], ref_out)
+                ref_acc[name] = {"errorL2": l2, "errorMaxRel": mx, "targets_checked": int(ref_rows.size)}
 
     extra = dict(variants=variants, accuracy_vs_oracle=accuracy, roofline=rl, cpu_baseline=cpu)
+    if not args.no_cpu_baseline and ref_cpu is not None:
+        extra.update(cpu_reference=ref_cpu, accuracy_vs_reference=ref_acc)
     if "per-leaf" in variants and head == "mixed":
         extra["speedup_vs_fp64_per_leaf_launch"] = hv["gpair_s"] / variants["per-leaf"]["gpair_s"]
     if "fp64" in variants and head == "mixed":

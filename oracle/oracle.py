@@ -86,6 +86,7 @@ def ref():
             raise RuntimeError(f"reference oracle not built: {REF_LIB_PATH}")
         L = ctypes.CDLL(REF_LIB_PATH)
         L.simplekernel_available.argtypes = [ctypes.c_uint, ctypes.c_int]
+        L.simplekernel_threads.argtypes = [ctypes.c_int]
         L.simplekernel_pair.argtypes = [ctypes.c_uint, ctypes.c_int, _dp, _dp, _dp, _dp]
         L.simplekernel_direct.argtypes = [ctypes.c_uint, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           _dp, _dp, ctypes.c_int, _dp, _dp]
@@ -202,6 +203,11 @@ def simplekernel_direct(kernel, layer, ks, kt, src_coord, src_value, trg_coord):
     if rc != 0:
         raise ValueError("simplekernel unavailable for this kernel/layer")
     return out.reshape(nt, kt)
+
+
+def simplekernel_threads(n=0):
+    """Set (n > 0) and return the OpenMP thread count of the reference direct sum."""
+    return ref().simplekernel_threads(int(n))
 
 
 def simplekernel_pair(kernel, layer, s, t, f):

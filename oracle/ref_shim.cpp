@@ -10,6 +10,8 @@
 // chosen by the KERNEL enum value and layer, as in the SL_kernels map (Test/Test.cpp:12-23).
 #include "SimpleKernel.hpp"
 
+#include <omp.h>
+
 typedef void (*kernel_func)(double *, double *, double *, double *);
 
 static kernel_func pick(unsigned kernel, int layer) {
@@ -34,6 +36,12 @@ extern "C" {
 
 int simplekernel_available(unsigned kernel, int layer) { return pick(kernel, layer) != nullptr; }
 
+// sets the OpenMP thread count when n > 0; returns the count simplekernel_direct uses
+int simplekernel_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+}
+
 // one pair: out[0..19] = zeroed then filled by the reference function
 int simplekernel_pair(unsigned kernel, int layer, const double *s, const double *t, const double *f, double *out) {
     kernel_func fn = pick(kernel, layer);
@@ -46,11 +54,13 @@ int simplekernel_pair(unsigned kernel, int layer, const double *s, const double 
     return 0;
 }
 
-// direct sum with the runSimpleKernel accumulation (Test/Test.cpp:478-510); trg_value +=
+// direct sum with the runSimpleKernel accumulation (Test/Test.cpp:478-510); trg_value +=.
+// Targets are split over OpenMP threads as in that loop (Test/Test.cpp:478, omp parallel for).
 int simplekernel_direct(unsigned kernel, int layer, int ks, int kt, int nsrc, const double *src_coord,
                         const double *src_value, int ntrg, const double *trg_coord, double *trg_value) {
     kernel_func fn = pick(kernel, layer);
     if (!fn || ks > 16 || kt > 20) return -1;
+#pragma omp parallel for
     for (int i = 0; i < ntrg; i++) {
         double t[3] = {trg_coord[3 * i], trg_coord[3 * i + 1], trg_coord[3 * i + 2]};
         for (int j = 0; j < nsrc; j++) {

@@ -1,6 +1,6 @@
-"""GPU parity through the C++ mirror of STKFMM's interface (include/stkfmm_p2p.hpp): the driver
-tests/cxx/direct_driver follows the reference test's direct mode (Test/Test.cpp:542-572) and every
-kernel's result is compared with the oracle on the driver's own inputs."""
+"""GPU parity through the C++ mirror of STKFMM's interface (include/stkfmm_p2p.hpp): the test
+program tests/cxx/direct_driver follows the reference test's direct mode (Test/Test.cpp:542-572) and every
+kernel's result is compared with the oracle on that program's own inputs."""
 import os
 import subprocess
 

@@ -119,3 +119,33 @@ def test_error_metrics_match_reference_definition():
     assert np.isclose(e["errorRMS"], np.sqrt(1.25) / np.sqrt(3))
     assert np.isclose(e["errorMaxRel"], 1.0 / 2.5)
     assert np.isclose(e["drift"], (0 - 0.5 + 1.0) / 3)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build oracle/_ref absent")
+@pytest.mark.parametrize("uk,parts", [(13, ((128, 0, 0, 4), (128, 1, 4, 13))), (11, ((16, 0, 0, 4),))])
+def test_ulist_leaves_match_live_reference(uk, parts):
+    """The oracle's U-list evaluation (leaf ranges of an adaptive octree, fused single + double layer
+    for PVel) equals the reference's SimpleKernel summed over each leaf's U-list sources.  Each
+    target sums ~10^4 sources in another order than the reference loop, so the bound is the FP64
+    parity bound of the path, 1e-12."""
+    import stkfmm_amd as S
+    from stkfmm_amd import workloads as W
+
+    x = W.pvel_points(12000)
+    tr = S.Tree(x, 400)
+    xs = x[tr.perm]
+    rng = np.random.default_rng(7)
+    v = source_values(uk, tr.n, rng)
+    leaves = np.unique(np.concatenate([rng.choice(tr.n_leaves, 3, replace=False),
+                                       [np.argmax(tr.leaf_end - tr.leaf_begin)]]))
+    got = O.ranges(uk, xs, v, xs, tr.leaf_begin, tr.leaf_end, tr.ulist_ptr, tr.rng_begin, tr.rng_end, sel=leaves,
+                   trg_value=np.zeros((tr.n, O.trg_dim(uk))))
+    kt = O.trg_dim(uk)
+    for leaf in leaves:
+        b, e = tr.leaf_begin[leaf], tr.leaf_end[leaf]
+        src = np.concatenate([np.arange(tr.rng_begin[r], tr.rng_end[r])
+                              for r in range(tr.ulist_ptr[leaf], tr.ulist_ptr[leaf + 1])])
+        ref = sum(O.simplekernel_direct(K, layer, c1 - c0, kt, xs[src], v[src, c0:c1], xs[b:e])
+                  for K, layer, c0, c1 in parts)
+        l2, mx = O.max_error(got[b:e], ref)
+        assert l2 < 1e-12 and mx < 1e-12, (leaf, l2, mx)
