@@ -250,15 +250,19 @@ template <> struct VjpOf<8> { using type = VjpStokesDoublePVel; };
 template <> struct VjpOf<11> { using type = VjpRpyULapU; };
 template <> struct VjpOf<13> { using type = VjpStokesPVelFused; };
 
-// lane i receives the value of lane i + 1 (lane 63 that of lane 0): DPP wave_rol:1
+// lane i receives the value of lane i + 1 (lane 63 that of lane 0): DPP wave_rol:1.  Called with all
+// 64 lanes active, so every lane reads an active lane and the "old" operand (the value a lane keeps
+// when its source lane is inactive) is never used; passing v itself lets the move rotate in place
+// instead of first zeroing a fresh register.
 __device__ __forceinline__ float rotate_down(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x134, 0xF, 0xF,
-                                                                 false));
+    const int b = __builtin_bit_cast(int, v);
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, 0x134, 0xF, 0xF, false));
 }
 __device__ __forceinline__ double rotate_down(double v) {
     const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, 0x134, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), 0x134, 0xF, 0xF, false);
+    const int lo0 = (int)(unsigned)b, hi0 = (int)(unsigned)(b >> 32);
+    const int lo = __builtin_amdgcn_update_dpp(lo0, lo0, 0x134, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(hi0, hi0, 0x134, 0xF, 0xF, false);
     return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
