@@ -369,20 +369,53 @@ def pvel_adjoint(args, torch, S, W):
         S.vjp_device(uk, d_src, d_val, d_trg, d_w)
     torch.cuda.synchronize()
     adj = (time.perf_counter() - t) / args.steps
+    for _ in range(args.warmup):
+        S.vjp_device(uk, d_src, d_val, d_trg, d_w, precision=S.PREC.MIXED)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        S.vjp_device(uk, d_src, d_val, d_trg, d_w, precision=S.PREC.MIXED)
+    torch.cuda.synchronize()
+    adj_mixed = (time.perf_counter() - t) / args.steps
     a = d_src.clone().requires_grad_(True)
     b = d_val.clone().requires_grad_(True)
     c = d_trg.clone().requires_grad_(True)
+    (p2p(uk, a, b, c) * d_w).sum().backward()  # warm-up (first-call allocations)
     torch.cuda.synchronize()
     t = time.perf_counter()
-    (p2p(uk, a, b, c) * d_w).sum().backward()
+    for _ in range(3):
+        (p2p(uk, a, b, c) * d_w).sum().backward()
     torch.cuda.synchronize()
-    ag = time.perf_counter() - t
+    ag = (time.perf_counter() - t) / 3
+    cpu = acc = None
+    if not args.no_cpu_baseline:
+        # the oracle's adjoint (FP64 C, OpenMP) on every source against the first m targets: timed as
+        # the CPU baseline and used as the checker of both GPU precisions on the same sub-problem
+        from oracle import oracle as O
+        nth = threads()
+        m = min(8192, len(trg))
+        t = time.perf_counter()
+        ref = O.direct_vjp(int(uk), src, val, trg[:m], w[:m], nthreads=nth)
+        cpu_s = time.perf_counter() - t
+        acc = {}
+        for name, prec in (("fp64", S.PREC.FP64), ("mixed", S.PREC.MIXED)):
+            got = S.vjp_device(uk, d_src, d_val, d_trg[:m].contiguous(), d_w[:m].contiguous(), precision=prec)
+            acc[name] = {}
+            for g, x, r in zip(("src_value", "src_coord", "trg_coord"), got, ref):
+                l2, mx = O.max_error(x.cpu().numpy(), r)
+                acc[name][g] = {"errorL2": l2, "errorMaxRel": mx}
+        cpu = {"value": len(src) * m / cpu_s / 1e9, "unit": "Gpair/s", "cores": nth, "kind": "port",
+               "sample": (f"{len(src)} sources x the first {m} targets ({len(src) * m:.3e} pairs, {cpu_s:.2f} s): "
+                          "the oracle's FP64 adjoint (target pass + source pass), OpenMP"),
+               "cpu_model": cpu_model()}
     line("Gpair-interactions/s Stokes PVel forward + adjoint (direct sum, FP64)", 2 * pairs / (fwd + adj) / 1e9,
          "Gpair/s", args, 1e3 * (fwd + adj), "f64",
          {"workload": "pvel-adjoint-65536x65536", "pairs": pairs, "precision": "fp64",
           "value_counts": "forward pairs + adjoint pairs"},
          forward_ms=1e3 * fwd, adjoint_ms=1e3 * adj, autograd_fwd_bwd_ms=1e3 * ag,
-         forward_gpair_s=pairs / fwd / 1e9, adjoint_gpair_s=pairs / adj / 1e9)
+         forward_gpair_s=pairs / fwd / 1e9, adjoint_gpair_s=pairs / adj / 1e9,
+         adjoint_mixed_ms=1e3 * adj_mixed, adjoint_mixed_gpair_s=pairs / adj_mixed / 1e9,
+         cpu_baseline=cpu, accuracy_vs_oracle=acc)
 
 
 def stokeslet_instances(args, torch, S, W):

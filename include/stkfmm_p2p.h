@@ -362,7 +362,7 @@ int stkfmm_wall_eval_device(unsigned kernel, int precision, int nSL, const doubl
  * Device AoS arrays.  Outputs are OVERWRITTEN: g_src_value [nSrc*ks], g_src_coord [nSrc*3],
  * g_trg_coord [nTrg*3]; any of them may be NULL to skip it.  For the fused kernel src_value holds
  * the 13 values [f(3), TrD, D(9)] of each point.  One pass over the pairs; deterministic.  The
- * environment variable STKFMM_VJP_WAVES_PER_CU (default 32) sets how many waves per compute unit the
+ * environment variable STKFMM_VJP_WAVES_PER_CU (default 64) sets how many waves per compute unit the
  * pass is split into (more: better balance, larger partial-sum workspace). */
 int stkfmm_p2p_vjp_device(int uker, int precision, int nSrc, const double *d_srcCoord, const double *d_srcValue,
                           int nTrg, const double *d_trgCoord, const double *d_w, double *d_g_src_value,

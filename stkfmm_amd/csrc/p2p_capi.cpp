@@ -1061,14 +1061,18 @@ extern "C" int stkfmm_p2p_vjp_device(int uker, int precision, int nSrc, const do
     if (rc)
         return rc;
     const int ks = uker_ks(uker);
-    // about 32 waves per CU over a (target blocks) x (source chunks) grid, close to square (measured on
-    // MI355X, 65,536^2 PVel: 8 waves per CU 11.7 ms FP64 / 9.8 ms mixed, 24: 11.3 / 8.5, 48: 11.1 /
-    // 8.4; T = 1 slower); partials kept within ~2 GiB by fewer, larger blocks for large point counts
+    // about 64 waves per CU over a (target blocks) x (source chunks) grid, close to square; partials
+    // kept within ~2 GiB by fewer, larger blocks for large point counts.  Measured on MI355X,
+    // 65,536^2 (ms, FP64 / mixed), waves per CU 16 / 20 / 24 / 32 / 40 / 48 / 64 / 96: PVel 10.7 /
+    // 12.0 / 11.3 / 11.6 / 10.7 / 11.1 / 10.4 / 10.7 and 7.5 / 8.5 / 7.9 / 8.2 / 7.6 / 7.7 / 7.1 / 7.3
+    // -- the blocks fill whole rounds of the resident waves at 16 and 64 (FP64: 4,096 and 16,384
+    // blocks on 4,096 wave slots) and leave a part-empty last round at 32 (86 x 86 blocks); the same
+    // order holds for fused PVel and RPY.  T = 1 is slower.
     VjpLayout l;
     l.T = (nTrg >= 16 * kWave) ? 2 : 1;
     const long long ttiles = std::max(1, (nTrg + kWave * l.T - 1) / (kWave * l.T));
     const long long stiles = std::max(1, (nSrc + kSrcTile - 1) / kSrcTile);
-    long long per_cu = 32;
+    long long per_cu = 64;
     if (const char *e = std::getenv("STKFMM_VJP_WAVES_PER_CU")) // tuning and tests: other layouts
         per_cu = std::max(1L, std::strtol(e, nullptr, 10));
     const long long waves = per_cu * n_cu;
