@@ -52,6 +52,23 @@ namespace stkp2p {
 #ifndef STKP2P_MIXED_HILO
 #define STKP2P_MIXED_HILO 0
 #endif
+// STKP2P_FAR_F32 (build option, default 1): in mixed mode, a source tile clear of the item's target
+// box by a margin forms its displacements in FP32 from coordinates taken relative to the box centre
+// c (rounded to FP32 once per target per item and once per source per tile) -- one FP32 subtraction
+// per axis instead of an FP64 subtraction and a conversion.  Each component of t - c and s - c is
+// rounded with relative error <= 2^-24, so a displacement component errs by at most
+// 2^-24 (|t - c| + |s - c| + r) (the last term: the FP32 subtraction's own rounding).  A tile is
+// taken only when every source s satisfies |h| + |s - c| <= STKP2P_FAR_RATIO * dist(s, box), with
+// |h| the box's half-diagonal (so |t - c| <= |h| and dist(s, box) <= r for every target t): then
+// the error is at most (STKP2P_FAR_RATIO + 1) 2^-24 r per component, 3.0e-7 r at the default 4.
+// Cancellation between nearby coordinates -- the reason the displacement is FP64 -- is confined
+// to the tiles that fail the test, which keep the FP64 displacement.
+#ifndef STKP2P_FAR_F32
+#define STKP2P_FAR_F32 1
+#endif
+#ifndef STKP2P_FAR_RATIO
+#define STKP2P_FAR_RATIO 4.0
+#endif
 #if STKP2P_MIN_WAVES_PER_EU > 0
 #define STKP2P_ITEM_ATTR __attribute__((amdgpu_waves_per_eu(STKP2P_MIN_WAVES_PER_EU, 8)))
 #else
@@ -75,7 +92,10 @@ __global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemA
     // all at immediate offsets from one base address), led by the hi/lo coordinates with HILO
     constexpr bool HILO = (PREC != 0) && (STKP2P_MIXED_HILO != 0);
     constexpr int KOFF = HILO ? 6 : 0;
-    constexpr int KLP = ((KL + KOFF + 3) / 4) * 4;
+    // relative FP32 coordinates of the far-tile form after the values
+    constexpr bool FAR = (PREC != 0) && !HILO && HasUnmaskedEval<K>::value && (STKP2P_FAR_F32 != 0);
+    constexpr int FOFF = KOFF + KL;
+    constexpr int KLP = ((KL + KOFF + (FAR ? 3 : 0) + 3) / 4) * 4;
     __shared__ double s_x[HILO ? 1 : kSrcTile], s_y[HILO ? 1 : kSrcTile], s_z[HILO ? 1 : kSrcTile];
     __shared__ double s_vd[PREC == 0 ? KL : 1][kSrcTile];
     __shared__ __attribute__((aligned(16))) float s_vf[PREC == 0 ? 1 : kSrcTile][KLP];
@@ -124,6 +144,25 @@ __global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemA
         TargetBox<PREC> box;
         if constexpr (CLEAR)
             box.build(tx, ty, tz);
+        // far-tile form: box centre, half-extent and half-diagonal (wave-uniform), targets relative
+        // to the centre in FP32
+        double fc[3] = {0.0, 0.0, 0.0}, fh[3] = {0.0, 0.0, 0.0}, fhn = 0.0;
+        float tfx[T], tfy[T], tfz[T];
+        if constexpr (FAR) {
+            // computed by the vector unit (no scalar FP64), then held in scalar registers
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                fc[d] = uniform_f64(0.5 * (box.lo[d] + box.hi[d]));
+                fh[d] = uniform_f64(0.5 * (box.hi[d] - box.lo[d]));
+            }
+            fhn = uniform_f64(__builtin_sqrt(fh[0] * fh[0] + fh[1] * fh[1] + fh[2] * fh[2]));
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                tfx[t] = (float)(tx[t] - fc[0]);
+                tfy[t] = (float)(ty[t] - fc[1]);
+                tfz[t] = (float)(tz[t] - fc[2]);
+            }
+        }
         double acc[T][KT];
 #pragma unroll
         for (int t = 0; t < T; ++t)
@@ -136,6 +175,7 @@ __global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemA
             for (int base = rg.x; base < rg.y; base += kSrcTile) {
                 const int n = (rg.y - base < kSrcTile) ? rg.y - base : kSrcTile;
                 bool near = false; // this lane's source may coincide with a target of the item
+                bool far = true;   // this lane's source (if any) passes the far-tile test
                 __syncthreads(); // the previous tile has been read by every lane
                 if (lane < n) {
                     const long long s = base + lane;
@@ -158,6 +198,19 @@ __global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemA
                     }
                     if constexpr (CLEAR)
                         near = box.near(x, y, z);
+                    if constexpr (FAR) {
+                        const double rx = x - fc[0], ry = y - fc[1], rz = z - fc[2];
+                        const double gx = __builtin_fmax(__builtin_fabs(rx) - fh[0], 0.0);
+                        const double gy = __builtin_fmax(__builtin_fabs(ry) - fh[1], 0.0);
+                        const double gz = __builtin_fmax(__builtin_fabs(rz) - fh[2], 0.0);
+                        const double lhs = fhn + __builtin_sqrt(rx * rx + ry * ry + rz * rz);
+                        // false for NaN coordinates (near is then true as well)
+                        constexpr double R2 = STKP2P_FAR_RATIO * STKP2P_FAR_RATIO;
+                        far = !near && lhs * lhs <= R2 * (gx * gx + gy * gy + gz * gz);
+                        s_vf[lane][FOFF] = (float)rx;
+                        s_vf[lane][FOFF + 1] = (float)ry;
+                        s_vf[lane][FOFF + 2] = (float)rz;
+                    }
                     double vst[KL];
                     stage_for<K, PREC>(vin, vst);
                     if constexpr (HILO) {
@@ -184,9 +237,10 @@ __global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemA
                 __syncthreads();
                 // sources of this tile: every lane takes all of them (split == 1, uniform loop with
                 // immediate LDS offsets) or every split-th one starting at its sub-lane
-                auto tile_pairs = [&](auto strided, auto clear) {
+                auto tile_pairs = [&](auto strided, auto clear, auto farc) {
                     constexpr bool STR = decltype(strided)::value;
                     constexpr bool MASK = !decltype(clear)::value;
+                    constexpr bool FARM = decltype(farc)::value;
                     const int j_first = STR ? sub : 0;
                     const int j_step = STR ? split : 1;
                     if (PREC == 0) {
@@ -219,7 +273,11 @@ __global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemA
                                 // displacement is needed before the values
                                 float hx, hy, hz, lx, ly, lz;
                                 double sx, sy, sz;
-                                if constexpr (HILO) {
+                                if constexpr (FARM) {
+                                    hx = s_vf[j][FOFF];
+                                    hy = s_vf[j][FOFF + 1];
+                                    hz = s_vf[j][FOFF + 2];
+                                } else if constexpr (HILO) {
                                     hx = s_vf[j][0];
                                     hy = s_vf[j][1];
                                     hz = s_vf[j][2];
@@ -237,7 +295,9 @@ __global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemA
                                     f[k] = s_vf[j][KOFF + k];
 #pragma unroll
                                 for (int t = 0; t < T; ++t) {
-                                    if constexpr (HILO)
+                                    if constexpr (FARM)
+                                        eval_pair<K, false>(part[t], tfx[t] - hx, tfy[t] - hy, tfz[t] - hz, f);
+                                    else if constexpr (HILO)
                                         eval_pair<K, MASK>(part[t], (thx[t] - hx) + (tlx[t] - lx),
                                                                 (thy[t] - hy) + (tly[t] - ly),
                                                                 (thz[t] - hz) + (tlz[t] - lz), f);
@@ -254,18 +314,25 @@ __global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemA
                         }
                     }
                 };
-                // uniform: no lane holds a source near the targets
+                // uniform: no lane holds a source near the targets; every source passes the far test
                 const bool clear = CLEAR && !__any(near);
+                const bool far_tile = FAR && clear && __all(far);
+                using F = std::false_type;
+                using Tr = std::true_type;
                 if (ls == 0) {
-                    if (clear)
-                        tile_pairs(std::false_type{}, std::true_type{});
+                    if (far_tile)
+                        tile_pairs(F{}, Tr{}, std::integral_constant<bool, FAR>{});
+                    else if (clear)
+                        tile_pairs(F{}, Tr{}, F{});
                     else
-                        tile_pairs(std::false_type{}, std::false_type{});
+                        tile_pairs(F{}, F{}, F{});
                 } else {
-                    if (clear)
-                        tile_pairs(std::true_type{}, std::true_type{});
+                    if (far_tile)
+                        tile_pairs(Tr{}, Tr{}, std::integral_constant<bool, FAR>{});
+                    else if (clear)
+                        tile_pairs(Tr{}, Tr{}, F{});
                     else
-                        tile_pairs(std::true_type{}, std::false_type{});
+                        tile_pairs(Tr{}, F{}, F{});
                 }
             }
         }
