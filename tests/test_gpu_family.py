@@ -35,9 +35,10 @@ def clustered_points(seed, n_uniform=4000, n_cluster=4000, sigma=0.01, n_tight=6
     return W.wrap(np.concatenate(parts))
 
 
-def run(uk, prec, bc, x, cap, seed):
+def run(uk, prec, bc, x, cap, seed, shift=0.0):
+    # the tree is built on x; the plan and the oracle see the coordinates translated by shift
     tr = S.Tree(x, cap)
-    xs = x[tr.perm]
+    xs = x[tr.perm] + shift
     v = family_values(uk, tr.n, np.random.default_rng(seed))
     plan = S.Plan.from_tree(tr, uk, prec, tune=dict(broadcast=bc))
     out = zeros(tr.n, O.trg_dim(uk))
@@ -78,6 +79,20 @@ def test_core_ulist_clustered_vs_oracle(uk, prec, bc):
     l2, mx = O.max_error(out, ref)
     tol = FP64_TOL if prec == 0 else MIXED_TOL
     assert l2 < tol and mx < tol, (l2, mx)
+
+
+@pytest.mark.parametrize("uk", [2, 5, 11, 13, 16, 18])
+def test_ulist_far_from_origin_mixed(uk):
+    # every coordinate translated by 1000.5: the mixed mode's FP64 displacement (tiles near the
+    # target box) and its box-centred FP32 displacement (far tiles) both have to absorb the offset;
+    # checked against the oracle on the same translated coordinates
+    say(f"U-list far from the origin uker={uk}")
+    x = clustered_points(2100 + uk)
+    out, ref, tr = run(uk, S.PREC.MIXED, 0, x, 120, 2200 + uk, shift=1000.5)
+    assert tr.max_level >= 5
+    assert np.all(np.isfinite(out))
+    l2, mx = O.max_error(out, ref)
+    assert l2 < MIXED_TOL and mx < MIXED_TOL, (l2, mx)
 
 
 @pytest.mark.parametrize("uk", FAMILY)
