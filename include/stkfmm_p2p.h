@@ -46,7 +46,9 @@ enum stkfmm_ppkernel { STKFMM_PPKERNEL_SLS2T = 1, STKFMM_PPKERNEL_DLS2T = 2, STK
 /* arithmetic mode of the HIP kernels (no reference counterpart: the reference is FP64 only) */
 enum stkfmm_precision {
     STKFMM_PREC_FP64 = 0,  /* IEEE FP64 throughout (matches the reference formulas to ~1e-15) */
-    STKFMM_PREC_MIXED = 1  /* FP64 displacement + accumulation, FP32 rsqrt (+1 Newton) and polynomial */
+    STKFMM_PREC_MIXED = 1  /* FP64 displacement + accumulation, FP32 rsqrt (+1 Newton) and polynomial;
+                              source tiles far from a work item's targets form the displacement in FP32
+                              from box-centred coordinates (error <= 5 * 2^-24 r per component) */
 };
 
 /* micro-kernels: one per uKerEval of the reference (pvfmm::Kernel<double>::Ker_t entries) */
