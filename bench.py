@@ -16,6 +16,8 @@ Other workloads (--workload), each printing its own JSON line:
   rpy-ulist             4M log-normal clustered points, leaf cap 2000, RPY, mixed (autotuned) and FP64
   pvel-adjoint          65,536 x 65,536 Stokes PVel forward + adjoint (gradients of sum_t w_t.u_t)
   stokeslet-instances   512 independent Stokeslet direct sums in one HIP-graph replay
+  pvel-fmm-lists        the rest of an FMM evaluation's P2P-shaped work on the 1M PVel tree: S2M, L2T,
+                        W-list (M2T) and X-list (S2L) surface evaluations, order-8 surfaces
 
   python bench.py [--workload W] [--steps K] [--warmup W] [--gpus 1]
 """
@@ -33,7 +35,7 @@ sys.path.insert(0, ROOT)
 PEAK_FP64 = 78.6e12   # MI355X vector FP64, FLOP/s
 PEAK_FP32 = 157.3e12  # MI355X vector FP32 (published, packed), FLOP/s
 # algorithmic FLOPs per pair and the FP64 share in mixed mode (displacement + accumulation)
-FLOPS = {"pvel_fused": (72, 7), "rpy": (56, 9), "stokeslet": (29, 6), "lapgrad": (20, 7)}
+FLOPS = {"pvel_fused": (72, 7), "pvel_sl": (35, 7), "rpy": (56, 9), "stokeslet": (29, 6), "lapgrad": (20, 7)}
 NOTE = ("VALU-bound; peak = MI355X published vector FP64 78.6 / FP32 157.3 TFLOP/s combined by the "
         "FP64/FP32 split of the algorithmic FLOPs; the library is compiled with packed FP32 disabled")
 
@@ -43,7 +45,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="pvel-ulist",
                     choices=["pvel-ulist", "pvel-ulist-4m", "lapgrad-direct", "rpy-ulist", "pvel-adjoint",
-                             "stokeslet-instances"])
+                             "stokeslet-instances", "pvel-fmm-lists"])
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU work budget of the baseline sample")
@@ -418,6 +420,105 @@ def pvel_adjoint(args, torch, S, W):
          cpu_baseline=cpu, accuracy_vs_oracle=acc)
 
 
+def pvel_fmm_lists(args, torch, S, W):
+    """S2M, L2T, W-list and X-list surface evaluations of PVel (the stokes_pvel sub-kernel of each,
+    StokesLayerKernel.hpp:44-50) on the 1M-point tree, order-8 surfaces (296 points): one plan per
+    list, mixed and FP64, default tuning."""
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    x = W.pvel_points()
+    tree = S.Tree(x, 1000)
+    xs = x[tree.perm]
+    setup_s = time.perf_counter() - t0
+    K, p = S.KERNEL.PVel, 8
+    uk = S.get_uker(K, S.PPKERNEL.L2T)
+    assert uk == S.get_s2m_uker(K) == S.get_m2t_uker(K) == S.get_s2l_uker(K)
+    ls, ns = S.LeafSurfaces(tree, p, S.RAD_OUTER), S.NodeSurfaces(tree, p)
+    cnt = ls.count
+    g = np.random.default_rng(2024)
+    v_pts = W.pvel_values(tree.n)[0]
+    dens = np.zeros((max(tree.n_leaves, tree.n_nodes) * cnt, 4))
+    dens[:, :3] = g.normal(size=(dens.shape[0], 3))
+    d_x, d_v, d_dens = (torch.from_numpy(a).to(dev) for a in (xs, v_pts, dens))
+    h_ls, h_ns = ls.coord.cpu().numpy(), ns.coord.cpu().numpy()
+    nl, nn = tree.n_leaves, tree.n_nodes
+    # work lists as the surface classes build them: (sources, values, targets, n_trg, groups, ranges)
+    lb = np.arange(nl, dtype=np.int64) * cnt
+    xn = np.nonzero(np.diff(tree.x_ptr) > 0)[0]
+    xb = xn.astype(np.int64) * cnt
+    wb = tree.w_node.astype(np.int64) * cnt
+    lists = {
+        "S2M": (ls.to_surface_plan, (d_x, d_v, ls.coord), (xs, v_pts, h_ls), nl * cnt,
+                (lb, lb + cnt, np.arange(nl + 1), tree.leaf_begin, tree.leaf_end)),
+        "L2T": (ls.from_surface_plan, (ls.coord, d_dens, d_x), (h_ls, dens[:nl * cnt], xs), tree.n,
+                (tree.leaf_begin, tree.leaf_end, np.arange(nl + 1), lb, lb + cnt)),
+        "W": (ns.w_plan, (ns.coord, d_dens, d_x), (h_ns, dens[:nn * cnt], xs), tree.n,
+              (tree.leaf_begin, tree.leaf_end, tree.w_ptr, wb, wb + cnt)),
+        "X": (ns.x_plan, (d_x, d_v, ns.coord), (xs, v_pts, h_ns), nn * cnt,
+              (xb, xb + cnt, np.concatenate([[0], np.cumsum(np.diff(tree.x_ptr)[xn])]),
+               tree.leaf_begin[tree.x_leaf], tree.leaf_end[tree.x_leaf])),
+    }
+    stream = torch.cuda.Stream()
+    per, results = {}, {}
+    for name, (mk, dev_args, _, n_trg, _) in lists.items():
+        for prec_name, prec in (("mixed", S.PREC.MIXED), ("fp64", S.PREC.FP64)):
+            plan = mk(uk, prec)
+            out = torch.zeros((n_trg, 4), dtype=torch.float64, device=dev)
+            kw = dict(src_coord=dev_args[0], src_value=dev_args[1], trg_coord=dev_args[2], trg_value=out)
+            wall, item_ms, reduce_ms, _ = Timed(torch, plan, kw, stream)(max(args.steps, 10), args.warmup)
+            steps = max(args.steps, 10)
+            pairs = plan.stats()["pairs"]
+            res = torch.zeros_like(out)
+            plan.execute(**dict(kw, trg_value=res))
+            torch.cuda.synchronize()
+            per.setdefault(name, {"pairs": pairs})[prec_name] = {
+                "ms_per_step": 1e3 * wall / steps, "gpair_s": pairs * steps / wall / 1e9, "item_ms": item_ms,
+                "reduce_ms": reduce_ms}
+            results[(name, prec_name)] = res.cpu().numpy()
+            print(f"# pvel-fmm-lists {name} {prec_name}: {pairs:.3e} pairs, {pairs * steps / wall / 1e9:.1f} Gpair/s, "
+                  f"item kernel {item_ms:.3f} ms", file=sys.stderr, flush=True)
+    total_pairs = sum(v["pairs"] for v in per.values())
+    ms_mixed = sum(v["mixed"]["ms_per_step"] for v in per.values())
+    ms_fp64 = sum(v["fp64"]["ms_per_step"] for v in per.values())
+    big = max(per, key=lambda k: per[k]["pairs"])
+    rl = roofline("pvel_sl", "mixed", per[big]["pairs"], per[big]["mixed"]["item_ms"],
+                  f"p2p_items_kernel<{int(uk)},MIXED,T=2> ({big} list)")
+    cpu = acc = None
+    if not args.no_cpu_baseline:
+        # the oracle over a sample of each list's groups: timed as the CPU baseline and used as the checker
+        from oracle import oracle as O
+        nth = threads()
+        acc, cpu_pairs, cpu_s = {}, 0, 0.0
+        for name, (_, _, host_args, n_trg, wl) in lists.items():
+            gb, ge, gp, rb, re = wl
+            sizes = np.asarray(ge, dtype=np.int64) - np.asarray(gb, dtype=np.int64)
+            csum = np.concatenate([[0], np.cumsum(np.asarray(re, dtype=np.int64) - np.asarray(rb, dtype=np.int64))])
+            cost = sizes * (csum[np.asarray(gp)[1:]] - csum[np.asarray(gp)[:-1]])
+            order = np.random.default_rng(7).permutation(len(gb))
+            sel = order[: int(np.searchsorted(np.cumsum(cost[order]), 1.5e9)) + 1]
+            ref = np.zeros((n_trg, 4))
+            t = time.perf_counter()
+            ref = O.ranges(int(uk), host_args[0], host_args[1], host_args[2], gb, ge, gp, rb, re, sel=sel,
+                           nthreads=nth, trg_value=ref)
+            cpu_s += time.perf_counter() - t
+            cpu_pairs += int(cost[sel].sum())
+            rows = np.concatenate([np.arange(gb[i], ge[i]) for i in sel])
+            acc[name] = {}
+            for prec_name in ("mixed", "fp64"):
+                l2, mx = O.max_error(results[(name, prec_name)][rows], ref[rows])
+                acc[name][prec_name] = {"errorL2": l2, "errorMaxRel": mx, "targets_checked": int(rows.size)}
+        cpu = {"value": cpu_pairs / cpu_s / 1e9, "unit": "Gpair/s", "cores": nth, "kind": "port",
+               "sample": (f"random groups of each list, {cpu_pairs:.3e} pairs in all, {cpu_s:.1f} s; the oracle's "
+                          "scalar FP64 restatement of stokes_pvel, OpenMP over groups"),
+               "cpu_model": cpu_model()}
+    line("Gpair-interactions/s Stokes PVel FMM surface lists (S2M + L2T + W + X), mixed",
+         total_pairs / (ms_mixed * 1e-3) / 1e9, "Gpair/s", args, ms_mixed, "f64+f32 (mixed)",
+         {"workload": "pvel-fmm-lists-1M-clustered-leafcap1000-p8", "points": tree.n, "leaves": nl, "nodes": nn,
+          "surface_points": cnt, "pairs": total_pairs, "precision": "mixed", "setup_s": setup_s},
+         lists=per, fp64_gpair_s=total_pairs / (ms_fp64 * 1e-3) / 1e9, roofline=rl, cpu_baseline=cpu,
+         accuracy_vs_oracle=acc)
+
+
 def stokeslet_instances(args, torch, S, W):
     dev = torch.device("cuda:0")
     sizes, offsets, coord, f = W.stokeslet_instances()
@@ -495,6 +596,8 @@ def main():
                     "rpy-ulist-4M-lognormal-leafcap2000", ["mixed", "fp64"])
     elif args.workload == "pvel-adjoint":
         pvel_adjoint(args, torch, S, W)
+    elif args.workload == "pvel-fmm-lists":
+        pvel_fmm_lists(args, torch, S, W)
     else:
         stokeslet_instances(args, torch, S, W)
 
