@@ -18,6 +18,8 @@ Other workloads (--workload), each printing its own JSON line:
   stokeslet-instances   512 independent Stokeslet direct sums in one HIP-graph replay
   pvel-fmm-lists        the rest of an FMM evaluation's P2P-shaped work on the 1M PVel tree: S2M, L2T,
                         W-list (M2T) and X-list (S2L) surface evaluations, order-8 surfaces
+  wall-images           StkWallFMM's no-slip wall image system (Stokes and RPY), 65,536 sources and
+                        targets above the wall, by direct sums
 
   python bench.py [--workload W] [--steps K] [--warmup W] [--gpus 1]
 """
@@ -45,7 +47,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="pvel-ulist",
                     choices=["pvel-ulist", "pvel-ulist-4m", "lapgrad-direct", "rpy-ulist", "pvel-adjoint",
-                             "stokeslet-instances", "pvel-fmm-lists"])
+                             "stokeslet-instances", "pvel-fmm-lists", "wall-images"])
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU work budget of the baseline sample")
@@ -519,6 +521,78 @@ def pvel_fmm_lists(args, torch, S, W):
          accuracy_vs_oracle=acc)
 
 
+def wall_images(args, torch, S, W):
+    """StkWallFMM's image system (Lib/src/StkWallFMM.cpp:154-292) on 65,536 sources and 65,536
+    targets in z in (0, 0.5) above the wall z = 0: Stokes (7 constituent direct sums of n x m pairs:
+    the Stokeslet, LapPGrad and LapPGradGrad over originals + images, the dipole sum over images) and
+    RPY (10: RPY, LapPGradGrad, LapPGrad and dipole sums over originals + images, the dipole and
+    quadrupole gradient sums over images); a pair is one evaluation in one constituent sum."""
+    dev = torch.device("cuda:0")
+    n = 65_536
+    g = np.random.default_rng(31)
+    x = g.random((n, 3))
+    x[:, 2] = g.uniform(0.0, 0.5, n)
+    t = g.random((n, 3))
+    t[:, 2] = g.uniform(0.0, 0.5, n)
+    res = {}
+    for kernel, name, sums, kt in ((S.KERNEL.Stokes, "stokes", 7, 3), (S.KERNEL.RPY, "rpy", 10, 6)):
+        f = g.normal(size=(n, 3))
+        if kernel == S.KERNEL.RPY:
+            f = np.concatenate([f, g.uniform(1e-3, 5e-3, (n, 1))], axis=1)
+        d_x, d_f, d_t = (torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in (x, f, t))
+        pairs = sums * n * n
+        entry = {"pairs": pairs, "constituent_sums": sums}
+        outs = {}
+        for prec_name, prec in (("mixed", S.PREC.MIXED), ("fp64", S.PREC.FP64)):
+            out = torch.zeros((n, kt), dtype=torch.float64, device=dev)
+            for _ in range(args.warmup):
+                S.wall_device(kernel, d_x, d_f, d_t, out, precision=prec)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                S.wall_device(kernel, d_x, d_f, d_t, out, precision=prec)
+            torch.cuda.synchronize()
+            wall = (time.perf_counter() - t0) / args.steps
+            o = torch.zeros((n, kt), dtype=torch.float64, device=dev)
+            S.wall_device(kernel, d_x, d_f, d_t, o, precision=prec)
+            torch.cuda.synchronize()
+            outs[prec_name] = o.cpu().numpy()
+            entry[prec_name] = {"ms_per_step": 1e3 * wall, "gpair_s": pairs / wall / 1e9}
+            print(f"# wall-images {name} {prec_name}: {pairs / wall / 1e9:.1f} Gpair/s, {1e3 * wall:.3f} ms",
+                  file=sys.stderr, flush=True)
+        res[name] = (entry, f, outs)
+    cpu = acc = None
+    if not args.no_cpu_baseline:
+        # the oracle's image system on every source against the first m targets: timed as the CPU
+        # baseline and used as the checker
+        from oracle import oracle as O
+        nth = threads()
+        m = 1024
+        acc, cpu_pairs, cpu_s = {}, 0, 0.0
+        for name, (entry, f, outs) in res.items():
+            kernel = 8 if name == "stokes" else 16
+            t0 = time.perf_counter()
+            ref = O.wall(kernel, x, f, t[:m], nthreads=nth)
+            cpu_s += time.perf_counter() - t0
+            cpu_pairs += entry["constituent_sums"] * n * m
+            acc[name] = {}
+            for prec_name, o in outs.items():
+                l2, mx = O.max_error(o[:m], ref)
+                acc[name][prec_name] = {"errorL2": l2, "errorMaxRel": mx, "targets_checked": m}
+        cpu = {"value": cpu_pairs / cpu_s / 1e9, "unit": "Gpair/s", "cores": nth, "kind": "port",
+               "sample": (f"both kernels, all {n} sources x the first {m} targets ({cpu_pairs:.3e} pairs, "
+                          f"{cpu_s:.1f} s); the oracle's scalar FP64 image system, OpenMP"),
+               "cpu_model": cpu_model()}
+    total = sum(e["pairs"] for e, _, _ in res.values())
+    ms_mixed = sum(e["mixed"]["ms_per_step"] for e, _, _ in res.values())
+    line("Gpair-interactions/s StkWallFMM wall images (Stokes + RPY direct sums), mixed", total / (ms_mixed * 1e-3) / 1e9,
+         "Gpair/s", args, ms_mixed, "f64+f32 (mixed)",
+         {"workload": "wall-images-65536x65536", "points": n, "pairs": total, "precision": "mixed"},
+         kernels={k: e for k, (e, _, _) in res.items()},
+         fp64_gpair_s=total / (sum(e["fp64"]["ms_per_step"] for e, _, _ in res.values()) * 1e-3) / 1e9,
+         cpu_baseline=cpu, accuracy_vs_oracle=acc)
+
+
 def stokeslet_instances(args, torch, S, W):
     dev = torch.device("cuda:0")
     sizes, offsets, coord, f = W.stokeslet_instances()
@@ -598,6 +672,8 @@ def main():
         pvel_adjoint(args, torch, S, W)
     elif args.workload == "pvel-fmm-lists":
         pvel_fmm_lists(args, torch, S, W)
+    elif args.workload == "wall-images":
+        wall_images(args, torch, S, W)
     else:
         stokeslet_instances(args, torch, S, W)
 
