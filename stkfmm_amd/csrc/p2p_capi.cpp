@@ -779,7 +779,8 @@ int direct_plan(int uk, int prec, int n_src, int n_trg, stkfmm_plan **out) {
 // grow-only device buffers for the host-pointer entries
 struct Workspace {
     int device = -1;
-    // slots 0-3: host-pointer direct sums; slot 4: adjoint partial sums; slot 5: wall images
+    // slot 0: host-pointer entries (inputs and output, packed); slots 1-3: unused; slot 4: adjoint
+    // partial sums; slot 5: wall images
     double *buf[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     size_t cap[6] = {0, 0, 0, 0, 0, 0};
 };
@@ -826,6 +827,57 @@ int eval_device_uk(int uk, int prec, int n_src, const double *sc, const double *
     return plan_enqueue(p, &a_sc, &a_sv, &a_sv2, &a_tc, &a_tv, nullptr, s, false);
 }
 
+// grow-only pinned host buffer through which the host-pointer entries move their data (one per
+// process, used under g_mu; every use ends with a synchronisation, so no copy is in flight when it
+// is reallocated)
+double *g_pinned = nullptr;
+size_t g_pinned_cap = 0;
+
+int pinned_get(size_t n, double **out) {
+    if (g_pinned_cap < n) {
+        if (g_pinned)
+            (void)hipHostFree(g_pinned);
+        g_pinned = nullptr;
+        g_pinned_cap = 0;
+        HIP_TRY(hipHostMalloc((void **)&g_pinned, sizeof(double) * std::max<size_t>(n, 1), hipHostMallocDefault),
+                "pinned staging buffer");
+        g_pinned_cap = n;
+    }
+    *out = g_pinned;
+    return STKFMM_OK;
+}
+
+// A host-pointer evaluation: the three inputs are packed into the pinned buffer and moved in one
+// copy; the device output starts at zero and comes back in one copy, after one synchronisation;
+// the caller's trgValue += result is then done here (the same additions as on the device, since
+// 0 + x is exact).  eval(d_sc, d_sv, d_tc, d_tv) enqueues the device work on the null stream.
+template <class Eval>
+int host_roundtrip(size_t ks, size_t kt, int n_src, const double *sc, const double *sv, int n_trg,
+                   const double *tc, double *tv, Eval eval) {
+    const size_t n_sc = 3 * (size_t)n_src, n_sv = ks * n_src, n_tc = 3 * (size_t)n_trg, n_tv = kt * n_trg;
+    const size_t n_in = n_sc + n_sv + n_tc;
+    double *h = nullptr, *d = nullptr;
+    int rc;
+    if ((rc = pinned_get(n_in + n_tv, &h)) || (rc = ws_get(0, n_in + n_tv, &d)))
+        return rc;
+    std::memcpy(h, sc, sizeof(double) * n_sc);
+    std::memcpy(h + n_sc, sv, sizeof(double) * n_sv);
+    std::memcpy(h + n_sc + n_sv, tc, sizeof(double) * n_tc);
+    double *d_tv = d + n_in, *h_tv = h + n_in;
+    HIP_TRY(hipMemcpyAsync(d, h, sizeof(double) * n_in, hipMemcpyHostToDevice, nullptr), "copy inputs");
+    HIP_TRY(hipMemsetAsync(d_tv, 0, sizeof(double) * n_tv, nullptr), "clear output");
+    rc = eval(d, d + n_sc, d + n_sc + n_sv, d_tv);
+    if (rc) {
+        (void)hipStreamSynchronize(nullptr);
+        return rc;
+    }
+    HIP_TRY(hipMemcpyAsync(h_tv, d_tv, sizeof(double) * n_tv, hipMemcpyDeviceToHost, nullptr), "copy result");
+    HIP_TRY(hipStreamSynchronize(nullptr), "synchronise");
+    for (size_t i = 0; i < n_tv; ++i)
+        tv[i] += h_tv[i];
+    return STKFMM_OK;
+}
+
 int eval_host_uk(int uk, int prec, int n_src, const double *sc, const double *sv, int n_trg, const double *tc,
                  double *tv) {
     if (n_src < 0 || n_trg < 0)
@@ -834,21 +886,10 @@ int eval_host_uk(int uk, int prec, int n_src, const double *sc, const double *sv
         return STKFMM_OK;
     if (!sc || !sv || !tc || !tv)
         return set_error(STKFMM_ERR_ARG, "NULL buffer");
-    const size_t ks = uker_ks(uk), kt = uker_kt(uk);
-    double *d_sc, *d_sv, *d_tc, *d_tv;
-    int rc;
-    if ((rc = ws_get(0, 3 * (size_t)n_src, &d_sc)) || (rc = ws_get(1, ks * n_src, &d_sv)) ||
-        (rc = ws_get(2, 3 * (size_t)n_trg, &d_tc)) || (rc = ws_get(3, kt * n_trg, &d_tv)))
-        return rc;
-    HIP_TRY(hipMemcpy(d_sc, sc, sizeof(double) * 3 * n_src, hipMemcpyHostToDevice), "copy srcCoord");
-    HIP_TRY(hipMemcpy(d_sv, sv, sizeof(double) * ks * n_src, hipMemcpyHostToDevice), "copy srcValue");
-    HIP_TRY(hipMemcpy(d_tc, tc, sizeof(double) * 3 * n_trg, hipMemcpyHostToDevice), "copy trgCoord");
-    HIP_TRY(hipMemcpy(d_tv, tv, sizeof(double) * kt * n_trg, hipMemcpyHostToDevice), "copy trgValue");
-    rc = eval_device_uk(uk, prec, n_src, d_sc, d_sv, n_trg, d_tc, d_tv, nullptr);
-    if (rc)
-        return rc;
-    HIP_TRY(hipMemcpy(tv, d_tv, sizeof(double) * kt * n_trg, hipMemcpyDeviceToHost), "copy trgValue back");
-    return STKFMM_OK;
+    return host_roundtrip(uker_ks(uk), uker_kt(uk), n_src, sc, sv, n_trg, tc, tv,
+                          [&](const double *d_sc, const double *d_sv, const double *d_tc, double *d_tv) {
+                              return eval_device_uk(uk, prec, n_src, d_sc, d_sv, n_trg, d_tc, d_tv, nullptr);
+                          });
 }
 
 int resolve(unsigned kernel, unsigned p2p, int precision, int *uk) {
@@ -983,19 +1024,10 @@ extern "C" int stkfmm_wall_evaluate(unsigned kernel, int precision, int nSL, con
         return set_error(STKFMM_ERR_ARG, "NULL buffer");
     const size_t ks = kernel == STKFMM_KERNEL_Stokes ? 3 : 4, kt = kernel == STKFMM_KERNEL_Stokes ? 3 : 6;
     std::lock_guard<std::mutex> lk(g_mu);
-    double *d_sc, *d_sv, *d_tc, *d_tv;
-    if ((rc = ws_get(0, 3 * (size_t)nSL, &d_sc)) || (rc = ws_get(1, ks * nSL, &d_sv)) ||
-        (rc = ws_get(2, 3 * (size_t)nTrg, &d_tc)) || (rc = ws_get(3, kt * nTrg, &d_tv)))
-        return rc;
-    HIP_TRY(hipMemcpy(d_sc, srcCoord, sizeof(double) * 3 * nSL, hipMemcpyHostToDevice), "copy srcCoord");
-    HIP_TRY(hipMemcpy(d_sv, srcValue, sizeof(double) * ks * nSL, hipMemcpyHostToDevice), "copy srcValue");
-    HIP_TRY(hipMemcpy(d_tc, trgCoord, sizeof(double) * 3 * nTrg, hipMemcpyHostToDevice), "copy trgCoord");
-    HIP_TRY(hipMemcpy(d_tv, trgValue, sizeof(double) * kt * nTrg, hipMemcpyHostToDevice), "copy trgValue");
-    rc = wall_device(kernel, precision, nSL, d_sc, d_sv, nTrg, d_tc, d_tv, nullptr);
-    if (rc)
-        return rc;
-    HIP_TRY(hipMemcpy(trgValue, d_tv, sizeof(double) * kt * nTrg, hipMemcpyDeviceToHost), "copy trgValue back");
-    return STKFMM_OK;
+    return host_roundtrip(ks, kt, nSL, srcCoord, srcValue, nTrg, trgCoord, trgValue,
+                          [&](const double *d_sc, const double *d_sv, const double *d_tc, double *d_tv) {
+                              return wall_device(kernel, precision, nSL, d_sc, d_sv, nTrg, d_tc, d_tv, nullptr);
+                          });
 }
 
 extern "C" int stkfmm_ker_eval(int uker, const double *r_src, int src_cnt, const double *v_src, int dof,
