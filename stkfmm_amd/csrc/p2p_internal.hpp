@@ -12,8 +12,11 @@ namespace stkp2p {
 constexpr int kWave = 64;
 constexpr int kSrcTile = 64;
 // mixed mode: FP32 partial sums over this many sources before they are added to the FP64
-// accumulator of the target
-constexpr int kMixedAccBlock = 16;
+// accumulator of the target (build option STKP2P_MIXED_ACC_BLOCK)
+#ifndef STKP2P_MIXED_ACC_BLOCK
+#define STKP2P_MIXED_ACC_BLOCK 16
+#endif
+constexpr int kMixedAccBlock = STKP2P_MIXED_ACC_BLOCK;
 static_assert(kSrcTile % kMixedAccBlock == 0 && kWave % kMixedAccBlock == 0,
               "source tiles and chunks hold whole partial-sum blocks");
 
