@@ -81,8 +81,22 @@ template <class K> struct MixedUnroll<K, decltype((void)K::MIXED_UNROLL)> {
     static constexpr int value = K::MIXED_UNROLL;
 };
 
+// minimum resident waves per SIMD asked of the compiler for the mixed-mode item kernel:
+// MIXED_MIN_WAVES of the micro-kernel, else 1 (no constraint).  The RPY U-list kernel asks for 4: the
+// compiler then fits 128 VGPRs (instead of 158, 3 waves) by spilling a few item-level values to
+// scratch outside the pair loops -- measured 42.0 -> 41.4 ms on the 4M RPY U-list (same box), same
+// results.  For the fused PVel the same request measured no gain.
+template <class K, class = void> struct MixedMinWaves { static constexpr int value = 1; };
+template <class K> struct MixedMinWaves<K, decltype((void)K::MIXED_MIN_WAVES)> {
+    static constexpr int value = K::MIXED_MIN_WAVES;
+};
+template <int UK, int PREC> struct ItemMinWaves {
+    static constexpr int value = (PREC != 0) ? MixedMinWaves<typename UkerOf<UK>::type>::value : 1;
+};
+
 template <int UK, int PREC, int T>
-__global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR p2p_items_kernel(ItemArgs a) {
+__global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR
+__attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_kernel(ItemArgs a) {
     using K = typename UkerOf<UK>::type;
     constexpr int KL = K::KL, KT = K::KT, KS = K::KS, KS1 = K::KS1;
     constexpr int TT = kWave * T;

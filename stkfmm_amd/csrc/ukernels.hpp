@@ -536,6 +536,7 @@ struct UkRpyU {
 struct UkRpyULapU {
     static constexpr int KS = 4, KS1 = 4, KL = 8, KT = 6;
     static constexpr bool UNMASKED_EVAL = true;
+    static constexpr int MIXED_MIN_WAVES = 4; // item kernel's mixed-mode occupancy request (measured)
     static constexpr unsigned SRC_SCALE_MASK = 0x8u;
     __host__ __device__ static constexpr int deg(int k) { return k < 3 ? 1 : 3; }
     __host__ __device__ static constexpr double post(int k) { return k < 3 ? 1.0 : -6.0; }
