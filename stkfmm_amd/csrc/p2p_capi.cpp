@@ -268,6 +268,9 @@ struct stkfmm_plan {
     std::vector<int> group_items; // mode 3: item index ranges per launched group (host)
     // scalar-broadcast variant: staged source records, one copy per level in [level_min, level_max]
     int broadcast = 0;
+    // launch variant of the LDS item kernel: precision, or kPrecMixedFar (mixed with the far-tile
+    // displacement) for tree-derived mixed work lists at T = 2
+    int kernel_prec = 0;
     int level_min = 0, n_copies = 0, rec_words = 0;
     double *d_recs = nullptr;
 };
@@ -358,10 +361,14 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
     int rc = device_cu_count(&n_cu);
     if (rc)
         return rc;
+    // the far-tile form pays where targets and sources are spatially sorted (groups with octree
+    // levels); for direct sums of unsorted points its extra registers only cost occupancy
+    const int kernel_prec = (precision == STKFMM_PREC_MIXED && grp_level && T == 2 && !broadcast) ? kPrecMixedFar
+                                                                                                 : precision;
     if (broadcast)
         HIP_TRY(items_sgpr_occupancy(uk, precision, T, &bpc), "occupancy query");
     else
-        HIP_TRY(items_occupancy(uk, precision, T, &bpc), "occupancy query");
+        HIP_TRY(items_occupancy(uk, kernel_prec, T, &bpc), "occupancy query");
     if (bpc < 1)
         bpc = 1;
     const int concurrent = n_cu * bpc;
@@ -517,6 +524,7 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
     if (persistent == 3)
         p->group_items = group_items;
     p->broadcast = broadcast;
+    p->kernel_prec = kernel_prec;
     if (broadcast) {
         int lmin = 0, lmax = 0;
         if (precision != STKFMM_PREC_FP64 && grp_level && n_groups > 0) {
@@ -602,7 +610,7 @@ static int plan_enqueue(stkfmm_plan *p, const stkfmm_array *sc, const stkfmm_arr
     }
     auto launch = [&](int grid, const ItemArgs &ia) {
         return p->broadcast ? launch_items_sgpr(p->uk, p->prec, p->T, grid, ia, p->d_recs, p->level_min, s)
-                            : launch_items(p->uk, p->prec, p->T, grid, ia, s);
+                            : launch_items(p->uk, p->kernel_prec, p->T, grid, ia, s);
     };
     if (p->persistent == 3) {
         // one launch per target group, in work-list order (the per-leaf-launch scheme)

@@ -133,7 +133,9 @@ int uker_kt(int uk);
 double uker_scale(int uk);
 bool uker_has_vjp(int uk);
 
-// launchers (host); return hipSuccess or the launch error
+// launchers (host); return hipSuccess or the launch error.  launch_items / items_occupancy also
+// take prec = kPrecMixedFar: the mixed-mode LDS item kernel with the far-tile displacement (T = 2)
+constexpr int kPrecMixedFar = 2;
 hipError_t launch_items(int uk, int prec, int T, int grid, const ItemArgs &a, hipStream_t s);
 hipError_t launch_reduce(const ReduceArgs &a, hipStream_t s);
 hipError_t launch_fill_int(int *p, int n, int value, hipStream_t s); // n <= 256

@@ -62,7 +62,11 @@ namespace stkp2p {
 // |h| the box's half-diagonal (so |t - c| <= |h| and dist(s, box) <= r for every target t): then
 // the error is at most (STKP2P_FAR_RATIO + 1) 2^-24 r per component, 3.0e-7 r at the default 4.
 // Cancellation between nearby coordinates -- the reason the displacement is FP64 -- is confined
-// to the tiles that fail the test, which keep the FP64 displacement.
+// to the tiles that fail the test, which keep the FP64 displacement.  The far-tile loop raises the
+// kernel's register count, which costs occupancy where no tile is far (direct sums of unsorted
+// points: 512 Stokeslet instances measured 42.5 -> 44.8 ms), so it is a separate instantiation
+// (FARB = true, mixed, T = 2) that plans use only for tree-derived work lists (groups with octree
+// levels, whose target and source ranges are spatially compact).
 #ifndef STKP2P_FAR_F32
 #define STKP2P_FAR_F32 1
 #endif
@@ -94,7 +98,7 @@ template <int UK, int PREC> struct ItemMinWaves {
     static constexpr int value = (PREC != 0) ? MixedMinWaves<typename UkerOf<UK>::type>::value : 1;
 };
 
-template <int UK, int PREC, int T>
+template <int UK, int PREC, int T, bool FARB = false>
 __global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR
 __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_kernel(ItemArgs a) {
     using K = typename UkerOf<UK>::type;
@@ -107,7 +111,7 @@ __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_ke
     constexpr bool HILO = (PREC != 0) && (STKP2P_MIXED_HILO != 0);
     constexpr int KOFF = HILO ? 6 : 0;
     // relative FP32 coordinates of the far-tile form after the values
-    constexpr bool FAR = (PREC != 0) && !HILO && HasUnmaskedEval<K>::value && (STKP2P_FAR_F32 != 0);
+    constexpr bool FAR = FARB && (PREC != 0) && !HILO && HasUnmaskedEval<K>::value && (STKP2P_FAR_F32 != 0);
     constexpr int FOFF = KOFF + KL;
     constexpr int KLP = ((KL + KOFF + (FAR ? 3 : 0) + 3) / 4) * 4;
     __shared__ double s_x[HILO ? 1 : kSrcTile], s_y[HILO ? 1 : kSrcTile], s_z[HILO ? 1 : kSrcTile];
@@ -375,15 +379,15 @@ __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_ke
 
 // ---- per-micro-kernel launch helpers -----------------------------------------------------------
 
-template <int UK, int PREC, int T>
+template <int UK, int PREC, int T, bool FARB = false>
 hipError_t launch_items_one(int grid, const ItemArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL((p2p_items_kernel<UK, PREC, T>), dim3(grid), dim3(kWave), 0, s, a);
+    hipLaunchKernelGGL((p2p_items_kernel<UK, PREC, T, FARB>), dim3(grid), dim3(kWave), 0, s, a);
     return hipGetLastError();
 }
 
-template <int UK, int PREC, int T>
+template <int UK, int PREC, int T, bool FARB = false>
 hipError_t occ_items_one(int *blocks) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, p2p_items_kernel<UK, PREC, T>, kWave, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, p2p_items_kernel<UK, PREC, T, FARB>, kWave, 0);
 }
 
 template <int UK, int PREC, int T>
@@ -409,15 +413,23 @@ hipError_t occ_sgpr_one(int *blocks) {
     default: return hipErrorInvalidValue;                                                          \
     }
 
+// the LDS item kernel also takes prec = kPrecMixedFar (mixed with the far-tile form, T = 2 only)
+#define STKP2P_ITEMS_SWITCH(FN, ...)                                                               \
+    if (prec == kPrecMixedFar)                                                                     \
+        return (T == 2) ? FN<UK, 1, 2, true>(__VA_ARGS__) : hipErrorInvalidValue;                 \
+    STKP2P_PT_SWITCH(FN, __VA_ARGS__)
+
 template <int UK>
 hipError_t launch_items_uk(int prec, int T, int grid, const ItemArgs &a, hipStream_t s) {
-    STKP2P_PT_SWITCH(launch_items_one, grid, a, s)
+    STKP2P_ITEMS_SWITCH(launch_items_one, grid, a, s)
 }
 
 template <int UK>
 hipError_t occ_items_uk(int prec, int T, int *blocks) {
-    STKP2P_PT_SWITCH(occ_items_one, blocks)
+    STKP2P_ITEMS_SWITCH(occ_items_one, blocks)
 }
+
+#undef STKP2P_ITEMS_SWITCH
 
 template <int UK>
 hipError_t launch_sgpr_uk(int prec, int T, int grid, const ItemArgs &a, const double *recs, int lmin,
