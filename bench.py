@@ -215,8 +215,10 @@ def ulist_bench(args, torch, S, kind, uker, points, cap, values_fn, kt, metric, 
     hv = variants[head]
     T = (hv["tune"] or {}).get("targets_per_lane", 2)
     prec_tag = "MIXED" if head == "mixed" else "FP64"
+    # tree-derived mixed plans at T = 2 run the far-tile instantiation of the item kernel
+    far = ",far-tile form" if head == "mixed" and T == 2 else ""
     rl = roofline(kind, "mixed" if head == "mixed" else "fp64", pairs, hv["item_ms"],
-                  f"p2p_items_kernel<{int(uker)},{prec_tag},T={T}>")
+                  f"p2p_items_kernel<{int(uker)},{prec_tag},T={T}{far}>")
     tfile = os.path.join(ROOT, "profiles", f"{workload_name}_{head}_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
