@@ -20,20 +20,20 @@ namespace stkp2p {
 
 
 // out(perm[i], k) += scale * sum over the tile's slots of partial[slot][k][i]; one workgroup per
-// tile, blockDim = 64*T.
+// (tile, output component k = blockIdx.y), blockDim = 64*T.  Each (target, k) sums its slots in slot
+// order, so the result does not depend on the grid shape.
 __global__ void __launch_bounds__(256) p2p_reduce_kernel(ReduceArgs a) {
     const Tile tl = a.tiles[blockIdx.x];
     const int i = threadIdx.x;
+    const int k = blockIdx.y;
     if (i >= tl.trg_count)
         return;
     const long long row = tl.trg_begin + i;
     const long long o = a.perm ? (long long)a.perm[row] : row;
-    for (int k = 0; k < a.kt; ++k) {
-        double s = 0.0;
-        for (int slot = tl.slot_begin; slot < tl.slot_end; ++slot)
-            s += a.partial[((long long)slot * a.kt + k) * a.tt + i];
-        a.out.p[o * a.out.s + k * a.out.fs] += a.scale * s;
-    }
+    double s = 0.0;
+    for (int slot = tl.slot_begin; slot < tl.slot_end; ++slot)
+        s += a.partial[((long long)slot * a.kt + k) * a.tt + i];
+    a.out.p[o * a.out.s + k * a.out.fs] += a.scale * s;
 }
 
 __global__ void fill_int_kernel(int *p, int n, int value) {
@@ -124,7 +124,7 @@ int record_words(int uk, int prec) {
 hipError_t launch_reduce(const ReduceArgs &a, hipStream_t s) {
     if (a.n_tiles <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(p2p_reduce_kernel, dim3(a.n_tiles), dim3(a.tt), 0, s, a);
+    hipLaunchKernelGGL(p2p_reduce_kernel, dim3(a.n_tiles, a.kt), dim3(a.tt), 0, s, a);
     return hipGetLastError();
 }
 
