@@ -126,7 +126,7 @@ def lib():
         # no device.  Loading torch first lets this library bind to torch's runtime instead.
         try:
             import torch  # noqa: F401
-        except ImportError:
+        except (ImportError, OSError):  # torch absent or its own libraries unloadable: not needed here
             pass
         _lib = _declare(ctypes.CDLL(LIB_PATH))
     return _lib
