@@ -186,16 +186,16 @@ static hipError_t launch_vjp_t(const VjpArgs &a, const VjpLayout &l, double *par
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
-    const dim3 gt((a.n_trg + 255) / 256), gs((a.n_src + 255) / 256);
+    const unsigned bt = (unsigned)((a.n_trg + 255) / 256), bs = (unsigned)((a.n_src + 255) / 256);
     if (a.g_tc)
-        hipLaunchKernelGGL(vjp_reduce_kernel, gt, dim3(256), 0, s, (const double *)tpart, a.n_trg, 3, l.n_chunks, 0,
-                           3, a.g_tc, 3, 0, a.scale);
+        hipLaunchKernelGGL(vjp_reduce_kernel, dim3(bt, 3), dim3(256), 0, s, (const double *)tpart, a.n_trg, 3,
+                           l.n_chunks, 0, 3, a.g_tc, 3, 0, a.scale);
     if (a.g_sc)
-        hipLaunchKernelGGL(vjp_reduce_kernel, gs, dim3(256), 0, s, (const double *)spart, a.n_src, ND, l.n_blocks, 0,
-                           3, a.g_sc, 3, 0, a.scale);
+        hipLaunchKernelGGL(vjp_reduce_kernel, dim3(bs, 3), dim3(256), 0, s, (const double *)spart, a.n_src, ND,
+                           l.n_blocks, 0, 3, a.g_sc, 3, 0, a.scale);
     if (a.g_sv)
-        hipLaunchKernelGGL(vjp_reduce_kernel, gs, dim3(256), 0, s, (const double *)spart, a.n_src, ND, l.n_blocks, 3,
-                           ND, a.g_sv, KS, 0, a.scale);
+        hipLaunchKernelGGL(vjp_reduce_kernel, dim3(bs, ND - 3), dim3(256), 0, s, (const double *)spart, a.n_src, ND,
+                           l.n_blocks, 3, ND, a.g_sv, KS, 0, a.scale);
     return hipGetLastError();
 }
 

@@ -382,18 +382,19 @@ __global__ void __launch_bounds__(kWave) vjp_pair_kernel(VjpPass a) {
     }
 }
 
-// out[i*ld + off + d] = scale * sum_c partial[c][d][i] for d in [d0, d1)
+// out[i*ld + off + d] = scale * sum_c partial[c][d][i] for d = d0 + blockIdx.y (grid.y = d1 - d0:
+// one block row per component, so small point counts still give enough blocks); chunks are summed
+// in chunk order, so the result does not depend on the grid shape
 __global__ void vjp_reduce_kernel(const double *partial, int n, int nd, int nchunk, int d0, int d1,
                                   double *out, int ld, int off, double scale) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || out == nullptr)
+    const int d = d0 + (int)blockIdx.y;
+    if (i >= n || out == nullptr || d >= d1)
         return;
-    for (int d = d0; d < d1; ++d) {
-        double s = 0.0;
-        for (int c = 0; c < nchunk; ++c)
-            s += partial[((long long)c * nd + d) * n + i];
-        out[(long long)i * ld + off + (d - d0)] = scale * s;
-    }
+    double s = 0.0;
+    for (int c = 0; c < nchunk; ++c)
+        s += partial[((long long)c * nd + d) * n + i];
+    out[(long long)i * ld + off + (d - d0)] = scale * s;
 }
 
 } // namespace stkp2p
