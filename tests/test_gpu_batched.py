@@ -214,19 +214,21 @@ def test_workload2_pvel_ulist_4m():
     assert l2 < MIXED_TOL and mx < MIXED_TOL, (l2, mx)
 
 
-def instance_plan(sizes, offsets, prec, uk=6):
+def instance_plan(sizes, offsets, prec, uk=6, tune=None):
     n = int(offsets[-1])
     gb, ge = offsets[:-1], offsets[1:]
     gp = np.arange(len(sizes) + 1)
-    return S.Plan(uk, prec, n, n, gb, ge, None, gp, gb, ge)
+    return S.Plan(uk, prec, n, n, gb, ge, None, gp, gb, ge, tune)
 
 
-def test_instances_and_graph_replay():
-    """Many independent Stokeslet direct sums in one launch; graph replays equal the eager result."""
+@pytest.mark.parametrize("bc", [0, 1], ids=["lds", "scalar"])
+def test_instances_and_graph_replay(bc):
+    """Many independent Stokeslet direct sums in one launch; graph replays equal the eager result
+    (with the scalar-broadcast staging the graph also holds the record-staging kernel)."""
     import torch
-    say("instances + graph")
+    say(f"instances + graph broadcast={bc}")
     sizes, offsets, coord, f = W.stokeslet_instances(n_inst=24, n_lo=100, n_hi=2000)
-    plan = instance_plan(sizes, offsets, 0)
+    plan = instance_plan(sizes, offsets, 0, tune=dict(broadcast=bc))
     d_c, d_f = to_dev(coord), to_dev(f)
     out = zeros(len(coord), 3)
     plan.execute(d_c, d_f, d_c, out)
