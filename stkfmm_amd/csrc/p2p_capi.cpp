@@ -771,17 +771,15 @@ int direct_plan(int uk, int prec, int n_src, int n_trg, stkfmm_plan **out) {
         }
     }
     const int gb = 0, ge = n_trg, gp[2] = {0, 1}, rb = 0, re = n_src;
-    // one target group, FP64: one target per lane below 32,768 targets (twice the waves for a small
-    // sum), the scalar-broadcast staging at or above it.  Measured on MI355X (item kernel): 8192^2
-    // LapPGrad 0.127 -> 0.100 ms at T = 1; 65,536^2 PVel 3.91 -> 3.77 ms with scalar broadcast.  Only
-    // the target-to-lane mapping and the staging change, not the order in which a target sums its
-    // sources.  Mixed direct sums keep the plan defaults.
+    // one target group: one target per lane below 32,768 targets (twice the waves for a small sum),
+    // and in FP64 the scalar-broadcast staging at or above it.  Measured on MI355X (item kernel):
+    // 8192^2 LapPGrad 0.127 -> 0.100 ms FP64 and 0.115 -> 0.096 ms mixed at T = 1; 65,536^2 PVel FP64
+    // 3.91 -> 3.77 ms with scalar broadcast (mixed: no tuning measurably better).  Only the
+    // target-to-lane mapping and the staging change, not the order in which a target sums its sources.
     stkfmm_tune tn{};
-    if (prec == STKFMM_PREC_FP64) {
-        const bool small = n_trg < 32768;
-        tn.targets_per_lane = small ? 1 : 2;
-        tn.broadcast = small ? 0 : 1;
-    }
+    const bool small = n_trg < 32768;
+    tn.targets_per_lane = small ? 1 : 2;
+    tn.broadcast = (!small && prec == STKFMM_PREC_FP64) ? 1 : 0;
     stkfmm_plan *p = nullptr;
     int rc = stkfmm_plan_create(&p, uk, prec, n_src, n_trg, 1, &gb, &ge, nullptr, gp, &rb, &re, &tn);
     if (rc)
