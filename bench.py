@@ -19,7 +19,7 @@ Other workloads (--workload), each printing its own JSON line:
   pvel-fmm-lists        the rest of an FMM evaluation's P2P-shaped work on the 1M PVel tree: S2M, L2T,
                         W-list (M2T) and X-list (S2L) surface evaluations, order-8 surfaces
   wall-images           StkWallFMM's no-slip wall image system (Stokes and RPY), 65,536 sources and
-                        targets above the wall, by direct sums
+                        targets above the wall, by direct sums, FP64
 
   python bench.py [--workload W] [--steps K] [--warmup W] [--gpus 1]
 """
@@ -57,8 +57,6 @@ def parse():
                     help="run one variant only (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device copy of known size (PMC calibration) and exit")
-    ap.add_argument("--valu-rates", action="store_true",
-                    help="run stkfmm_amd/valu_rates (VALU issue rate per instruction kind) and exit")
     return ap.parse_args()
 
 
@@ -90,7 +88,19 @@ def cpu_model():
 
 
 def threads():
-    return max(1, min(16, os.cpu_count() or 1))
+    """Host threads of the CPU baseline: OMP_NUM_THREADS when the environment sets it (the host's
+    CPU share allotted to this GPU), else every core this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return max(1, len(os.sched_getaffinity(0)))
+
+
+def host_info():
+    """What the CPU baseline ran on: model, logical CPUs of the host, CPUs this process may use,
+    and where the thread count came from."""
+    return {"cpu_model": cpu_model(), "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "threads_from": "OMP_NUM_THREADS" if os.environ.get("OMP_NUM_THREADS", "").isdigit() else "affinity"}
 
 
 def line(metric, value, unit, args, ms_per_step, dtype, config, **extra):
@@ -266,7 +276,7 @@ def ulist_bench(args, torch, S, kind, uker, points, cap, values_fn, kt, metric, 
                           f"{sample_pairs:.3e} pairs, {cpu_s:.1f} s; FP64 C restatement of the reference uKerEval "
                           "algebra, AVX2 lanes over targets, OpenMP over leaves (PVFMM SIMD P2P not buildable "
                           "offline)"),
-               "cpu_model": cpu_model()}
+               **host_info()}
         accuracy = {}
         for name, res in results.items():
             l2, mx = O.max_error(res.cpu().numpy()[rows], ref[rows])
@@ -274,7 +284,7 @@ def ulist_bench(args, torch, S, kind, uker, points, cap, values_fn, kt, metric, 
         # the reference's own CPU code on a sample of the same leaves, timed and used as a second checker
         ref_cpu, ref_rows, ref_out = reference_sample(O, uker, tree, xs, allv, kt, sel, args.cpu_seconds / 2, nth)
         if ref_cpu is not None:
-            ref_cpu["cpu_model"] = cpu_model()
+            ref_cpu.update(host_info())
             ref_acc = {}
             for name, res in results.items():
                 l2, mx = O.max_error(res.cpu().numpy()[ref_rows], ref_out)
@@ -343,7 +353,7 @@ def lapgrad_direct(args, torch, S, W):
                           f"static(1): {len(trg) // 4000 + 1} busy threads) {chunked_s:.2f} s; all-core "
                           f"target split {all_s:.2f} s = {pairs / all_s / 1e9:.3f} Gpair/s; scalar FP64 C "
                           "restatement (PVFMM SIMD P2P not buildable offline)"),
-               "cpu_model": cpu_model()}
+               **host_info()}
     line("Gpair-interactions/s Laplace PGrad direct sum (evaluateKernel), FP64", pairs * steps / wall / 1e9,
          "Gpair/s", args, 1e3 * wall / steps, "f64",
          {"workload": "lapgrad-direct-8192x8192", "pairs": pairs, "precision": "fp64"},
@@ -413,7 +423,7 @@ def pvel_adjoint(args, torch, S, W):
         cpu = {"value": len(src) * m / cpu_s / 1e9, "unit": "Gpair/s", "cores": nth, "kind": "port",
                "sample": (f"{len(src)} sources x the first {m} targets ({len(src) * m:.3e} pairs, {cpu_s:.2f} s): "
                           "the oracle's FP64 adjoint (target pass + source pass), OpenMP"),
-               "cpu_model": cpu_model()}
+               **host_info()}
     line("Gpair-interactions/s Stokes PVel forward + adjoint (direct sum, FP64)", 2 * pairs / (fwd + adj) / 1e9,
          "Gpair/s", args, 1e3 * (fwd + adj), "f64",
          {"workload": "pvel-adjoint-65536x65536", "pairs": pairs, "precision": "fp64",
@@ -514,7 +524,7 @@ def pvel_fmm_lists(args, torch, S, W):
         cpu = {"value": cpu_pairs / cpu_s / 1e9, "unit": "Gpair/s", "cores": nth, "kind": "port",
                "sample": (f"random groups of each list, {cpu_pairs:.3e} pairs in all, {cpu_s:.1f} s; the oracle's "
                           "scalar FP64 restatement of stokes_pvel, OpenMP over groups"),
-               "cpu_model": cpu_model()}
+               **host_info()}
     line("Gpair-interactions/s Stokes PVel FMM surface lists (S2M + L2T + W + X), mixed",
          total_pairs / (ms_mixed * 1e-3) / 1e9, "Gpair/s", args, ms_mixed, "f64+f32 (mixed)",
          {"workload": "pvel-fmm-lists-1M-clustered-leafcap1000-p8", "points": tree.n, "leaves": nl, "nodes": nn,
@@ -545,7 +555,9 @@ def wall_images(args, torch, S, W):
         pairs = sums * n * n
         entry = {"pairs": pairs, "constituent_sums": sums}
         outs = {}
-        for prec_name, prec in (("mixed", S.PREC.MIXED), ("fp64", S.PREC.FP64)):
+        # FP64 only: the library refuses mixed precision for the wall images (the image sums cancel
+        # near the wall, which would amplify FP32 rounding beyond 1e-6)
+        for prec_name, prec in (("fp64", S.PREC.FP64),):
             out = torch.zeros((n, kt), dtype=torch.float64, device=dev)
             for _ in range(args.warmup):
                 S.wall_device(kernel, d_x, d_f, d_t, out, precision=prec)
@@ -584,15 +596,13 @@ def wall_images(args, torch, S, W):
         cpu = {"value": cpu_pairs / cpu_s / 1e9, "unit": "Gpair/s", "cores": nth, "kind": "port",
                "sample": (f"both kernels, all {n} sources x the first {m} targets ({cpu_pairs:.3e} pairs, "
                           f"{cpu_s:.1f} s); the oracle's scalar FP64 image system, OpenMP"),
-               "cpu_model": cpu_model()}
+               **host_info()}
     total = sum(e["pairs"] for e, _, _ in res.values())
-    ms_mixed = sum(e["mixed"]["ms_per_step"] for e, _, _ in res.values())
-    line("Gpair-interactions/s StkWallFMM wall images (Stokes + RPY direct sums), mixed", total / (ms_mixed * 1e-3) / 1e9,
-         "Gpair/s", args, ms_mixed, "f64+f32 (mixed)",
-         {"workload": "wall-images-65536x65536", "points": n, "pairs": total, "precision": "mixed"},
-         kernels={k: e for k, (e, _, _) in res.items()},
-         fp64_gpair_s=total / (sum(e["fp64"]["ms_per_step"] for e, _, _ in res.values()) * 1e-3) / 1e9,
-         cpu_baseline=cpu, accuracy_vs_oracle=acc)
+    ms = sum(e["fp64"]["ms_per_step"] for e, _, _ in res.values())
+    line("Gpair-interactions/s StkWallFMM wall images (Stokes + RPY direct sums), FP64", total / (ms * 1e-3) / 1e9,
+         "Gpair/s", args, ms, "f64",
+         {"workload": "wall-images-65536x65536", "points": n, "pairs": total, "precision": "fp64"},
+         kernels={k: e for k, (e, _, _) in res.items()}, cpu_baseline=cpu, accuracy_vs_oracle=acc)
 
 
 def stokeslet_instances(args, torch, S, W):
@@ -638,12 +648,6 @@ def main():
                                    "(multi-GPU sharding is a design note, not a measured mode)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
-    if args.valu_rates:
-        import subprocess
-        exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "stkfmm_amd", "valu_rates")
-        if not os.path.exists(exe):
-            sys.exit("stkfmm_amd/valu_rates is not built (make -C stkfmm_amd/csrc)")
-        sys.exit(subprocess.call([exe]))
     import torch
 
     import stkfmm_amd as S

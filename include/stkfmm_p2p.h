@@ -347,7 +347,9 @@ int stkfmm_scale_trg_device(unsigned kernel, int nTrg, double *d_trgValue, doubl
  * StkWallFMM::setPoints + evaluateFMM (:40-128) for a direct evaluation: the wall is the plane z = 0
  * of the caller's coordinates (the image of (x, y, z) is (x, y, -z)); sources and targets are expected
  * at z >= 0; no periodicity.  Stokes: source f(3) -> u(3); RPY: source [f(3), b] -> [u(3), lap u(3)].
- * AoS arrays; other kernels: STKFMM_ERR_KERNEL ("Kernel not supported", :125-127). */
+ * AoS arrays; other kernels: STKFMM_ERR_KERNEL ("Kernel not supported", :125-127).  FP64 only:
+ * precision STKFMM_PREC_MIXED returns STKFMM_ERR_UNSUPPORTED, because the image sums cancel the
+ * direct sums near the wall (u = 0 on it) and amplify FP32 rounding beyond the mixed mode's 1e-6. */
 int stkfmm_wall_evaluate(unsigned kernel, int precision, int nSL, const double *srcCoord, const double *srcValue,
                          int nTrg, const double *trgCoord, double *trgValue);
 int stkfmm_wall_eval_device(unsigned kernel, int precision, int nSL, const double *d_srcCoord,

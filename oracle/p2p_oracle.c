@@ -12,9 +12,12 @@
  *   - the PVFMM GenericKernel driver semantics as used by STKFMM: r = x_trg - x_src, masked
  *     r2 > 0, per-target sum over all sources in order, then trg += ScaleFactor * sum
  *     (inferred from Lib/src/FMMData.cpp:261-282 and the ScaleFactor<Real>() members);
- *   - FMMData::evaluateKernel's target chunking (Lib/src/FMMData.cpp:252-283) for timing;
- *   - hand-derived vector-Jacobian products (adjoints) of stokes_pvel, stokes_doublepvel and
- *     rpy_ulapu (no reference counterpart; pinned by finite differences in tests/).
+ *   - FMMData::evaluateKernel's target chunking (Lib/src/FMMData.cpp:252-283) for timing
+ *     (oracle_direct_chunked), and a range-list driver with targets in AVX2 lanes
+ *     (oracle_ranges_simd) for the timed CPU baseline; both equal their scalar drivers
+ *     (oracle_direct, oracle_ranges) bit for bit (tests/test_oracle_drivers.py);
+ *   - hand-derived vector-Jacobian products (adjoints) of ten micro-kernels (no reference
+ *     counterpart; pinned by finite differences in tests/).
  *
  * Pinning (see tests/test_oracle_*.py):
  *   - every forward micro-kernel is checked against the reference's own scalar oracle
@@ -809,13 +812,22 @@ int oracle_direct_chunked(int uk, int nsrc, const double *src_coord, const doubl
 
 /* ---- vectorised CPU restatement used as the timed CPU baseline ---------------------------------
  * Same per-pair algebra as above, laid out like PVFMM's SIMD driver: a block of targets in SIMD
- * lanes, sources streamed one at a time.  Per target the summation order over sources is the same
- * as oracle_ranges, so results agree with it to the last bit when no FMA contraction is applied.
- * Only the kernels that the baseline times are provided. */
+ * lanes, sources streamed one at a time.  Each block function continues the per-target sums it is
+ * given in acc ([KT][VB], one running sum per target lane) over one source range and hands them
+ * back, so a target's sum runs over all ranges of its group, source by source, in the order of
+ * oracle_ranges, with the same IEEE operations (no FMA contraction): results equal oracle_ranges
+ * bit for bit (tests/test_oracle_drivers.py).  Only the kernels that the baseline times are
+ * provided. */
 #define VB 8
-static void block_pvel_fused(int nt, const double *tx, const double *ty, const double *tz, int s0, int s1,
-                             const double *sc, const double *sv, double *out /* [4][VB] */) {
-    double a0[VB] = {0}, a1[VB] = {0}, a2[VB] = {0}, a3[VB] = {0};
+static void block_pvel_fused(const double *tx, const double *ty, const double *tz, int s0, int s1,
+                             const double *sc, const double *sv, double *acc /* [4][VB] */) {
+    double a0[VB], a1[VB], a2[VB], a3[VB];
+    for (int i = 0; i < VB; i++) {
+        a0[i] = acc[i];
+        a1[i] = acc[VB + i];
+        a2[i] = acc[2 * VB + i];
+        a3[i] = acc[3 * VB + i];
+    }
     for (int s = s0; s < s1; s++) {
         const double sx = sc[3 * s], sy = sc[3 * s + 1], sz = sc[3 * s + 2];
         const double *f = sv + 13 * (size_t)s;
@@ -851,18 +863,23 @@ static void block_pvel_fused(int nt, const double *tx, const double *ty, const d
             a3[i] += dz * q;
         }
     }
-    (void)nt;
     for (int i = 0; i < VB; i++) {
-        out[i] += a0[i];
-        out[VB + i] += a1[i];
-        out[2 * VB + i] += a2[i];
-        out[3 * VB + i] += a3[i];
+        acc[i] = a0[i];
+        acc[VB + i] = a1[i];
+        acc[2 * VB + i] = a2[i];
+        acc[3 * VB + i] = a3[i];
     }
 }
 
-static void block_lap_pgrad(int nt, const double *tx, const double *ty, const double *tz, int s0, int s1,
-                            const double *sc, const double *sv, double *out) {
-    double a0[VB] = {0}, a1[VB] = {0}, a2[VB] = {0}, a3[VB] = {0};
+static void block_lap_pgrad(const double *tx, const double *ty, const double *tz, int s0, int s1,
+                            const double *sc, const double *sv, double *acc /* [4][VB] */) {
+    double a0[VB], a1[VB], a2[VB], a3[VB];
+    for (int i = 0; i < VB; i++) {
+        a0[i] = acc[i];
+        a1[i] = acc[VB + i];
+        a2[i] = acc[2 * VB + i];
+        a3[i] = acc[3 * VB + i];
+    }
     for (int s = s0; s < s1; s++) {
         const double sx = sc[3 * s], sy = sc[3 * s + 1], sz = sc[3 * s + 2];
         const double q = sv[s];
@@ -878,19 +895,18 @@ static void block_lap_pgrad(int nt, const double *tx, const double *ty, const do
             a3[i] -= v * dz;
         }
     }
-    (void)nt;
     for (int i = 0; i < VB; i++) {
-        out[i] += a0[i];
-        out[VB + i] += a1[i];
-        out[2 * VB + i] += a2[i];
-        out[3 * VB + i] += a3[i];
+        acc[i] = a0[i];
+        acc[VB + i] = a1[i];
+        acc[2 * VB + i] = a2[i];
+        acc[3 * VB + i] = a3[i];
     }
 }
 
-static void block_rpy_ulapu(int nt, const double *tx, const double *ty, const double *tz, int s0, int s1,
-                            const double *sc, const double *sv, double *out) {
+static void block_rpy_ulapu(const double *tx, const double *ty, const double *tz, int s0, int s1,
+                            const double *sc, const double *sv, double *acc /* [6][VB] */) {
     double a[6][VB];
-    memset(a, 0, sizeof(a));
+    memcpy(a, acc, sizeof(a));
     for (int s = s0; s < s1; s++) {
         const double sx = sc[3 * s], sy = sc[3 * s + 1], sz = sc[3 * s + 2];
         const double *f = sv + 4 * (size_t)s;
@@ -915,14 +931,11 @@ static void block_rpy_ulapu(int nt, const double *tx, const double *ty, const do
             a[5][i] += 2.0 * cz;
         }
     }
-    (void)nt;
-    for (int k = 0; k < 6; k++)
-        for (int i = 0; i < VB; i++)
-            out[k * VB + i] += a[k][i];
+    memcpy(acc, a, sizeof(a));
 }
 
-typedef void (*block_fn)(int, const double *, const double *, const double *, int, int, const double *,
-                         const double *, double *);
+typedef void (*block_fn)(const double *, const double *, const double *, int, int, const double *, const double *,
+                         double *);
 
 /* Vectorised range-list driver for the CPU baseline: same arguments as oracle_ranges; supports
  * UK_STOKES_PVEL_FUSED, UK_LAPLACE_PGRAD and UK_RPY_ULAPU.  Returns -2 for other kernels. */
@@ -949,19 +962,20 @@ int oracle_ranges_simd(int uk, const double *src_coord, const double *src_value,
         const int g = sel ? sel[i] : i;
         for (int t0 = grp_trg_begin[g]; t0 < grp_trg_end[g]; t0 += VB) {
             const int nt = (grp_trg_end[g] - t0 < VB) ? grp_trg_end[g] - t0 : VB;
-            double tx[VB], ty[VB], tz[VB], out[16 * VB];
+            /* lanes past the group's last target repeat its first target; their sums are dropped */
+            double tx[VB], ty[VB], tz[VB], acc[16 * VB];
             for (int j = 0; j < VB; j++) {
                 const int t = t0 + (j < nt ? j : 0);
                 tx[j] = trg_coord[3 * t];
                 ty[j] = trg_coord[3 * t + 1];
                 tz[j] = trg_coord[3 * t + 2];
             }
-            memset(out, 0, sizeof(out));
+            memset(acc, 0, sizeof(acc));
             for (int q = grp_rng_ptr[g]; q < grp_rng_ptr[g + 1]; q++)
-                fn(nt, tx, ty, tz, rng_src_begin[q], rng_src_end[q], src_coord, src_value, out);
+                fn(tx, ty, tz, rng_src_begin[q], rng_src_end[q], src_coord, src_value, acc);
             for (int j = 0; j < nt; j++)
                 for (int k = 0; k < kt; k++)
-                    trg_value[(size_t)kt * (t0 + j) + k] += scale * out[k * VB + j];
+                    trg_value[(size_t)kt * (t0 + j) + k] += scale * acc[k * VB + j];
         }
     }
     return 0;

@@ -342,7 +342,8 @@ def _wall_dims(kernel):
 def wall_evaluate(kernel, src_coord, src_value, trg_coord, trg_value, precision=PREC.FP64):
     """StkWallFMM's no-slip wall image system (Lib/src/StkWallFMM.cpp:154-292) by direct sums, host
     arrays: trg_value += u.  The wall is z = 0; KERNEL.Stokes (f -> u) or KERNEL.RPY ([f, b] ->
-    [u, lap u])."""
+    [u, lap u]).  FP64 only: PREC.MIXED raises ``Unsupported`` (the image sums cancel near the wall,
+    which would amplify FP32 rounding beyond the mixed mode's 1e-6 bound)."""
     ks, kt = _wall_dims(kernel)
     ns = np.asarray(src_coord).reshape(-1, 3).shape[0]
     nt = np.asarray(trg_coord).reshape(-1, 3).shape[0]

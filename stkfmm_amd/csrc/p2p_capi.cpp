@@ -1007,7 +1007,14 @@ int wall_device(unsigned kernel, int prec, int n, const double *sc, const double
 int wall_check(unsigned kernel, int precision, int n, int m) {
     if (kernel != STKFMM_KERNEL_Stokes && kernel != STKFMM_KERNEL_RPY)
         return set_error(STKFMM_ERR_KERNEL, "Kernel not supported"); // StkWallFMM.cpp:125-127
-    if (precision != STKFMM_PREC_FP64 && precision != STKFMM_PREC_MIXED)
+    if (precision == STKFMM_PREC_MIXED)
+        // the assembled velocity cancels the image sums (u = 0 on the wall), which amplifies the FP32
+        // rounding of each constituent sum (~1e-7 of its magnitude) to ~1e-5 of the result: above the
+        // 1e-6 bound of the mixed mode, so the wall images are FP64 only
+        return set_error(STKFMM_ERR_UNSUPPORTED,
+                         "wall images: mixed precision is not supported (the image sums cancel near the wall "
+                         "and the FP32 evaluation would exceed 1e-6 relative error); use FP64");
+    if (precision != STKFMM_PREC_FP64)
         return set_error(STKFMM_ERR_ARG, "unknown precision %d", precision);
     if (n < 0 || m < 0 || n > (1 << 29))
         return set_error(STKFMM_ERR_ARG, "bad point count");

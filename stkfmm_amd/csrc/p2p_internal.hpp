@@ -89,7 +89,7 @@ struct ReduceArgs {
     double scale;         // ScaleFactor of the kernel
 };
 
-// adjoint of a direct sum (target-centric and source-centric passes)
+// adjoint of a direct sum (one pass over the pairs, vjp_kernels.hpp)
 struct VjpArgs {
     const double *sc;  // AoS source coords [n_src*3]
     const double *sv;  // AoS source values [n_src*KS]

@@ -5,7 +5,8 @@
 //       Stokeslet, RPY; the rest of the family (14..24) is instantiated in p2p_family.hip.
 //   p2p_reduce_kernel  per target: sum of its tile's partial slots in slot order (deterministic),
 //       times the scale factor, added (+=) into the caller's output (optionally permuted).
-//   vjp kernels        adjoint of a direct sum (target-centric and source-centric passes).
+//   vjp kernels        adjoint of a direct sum: one pass over the pairs gives the target-position,
+//                      source-position and source-value gradients (vjp_kernels.hpp).
 //   The launch entries below dispatch through the per-micro-kernel UkerOps table.
 #include <hip/hip_runtime.h>
 

@@ -113,6 +113,20 @@ def test_host_argument_checks():
         S.evaluate_kernel(S.KERNEL.RPY, 0, S.PPKERNEL.DLS2T, src, np.zeros((10, 4)), trg, np.zeros((6, 6)))
 
 
+@pytest.mark.parametrize("kernel,ks,kt", [(S.KERNEL.Stokes, 3, 3), (S.KERNEL.RPY, 4, 6)])
+def test_wall_images_refuse_mixed_precision(kernel, ks, kt):
+    # the image sums cancel near the wall: FP32 evaluation would miss the mixed mode's 1e-6, so the
+    # entry refuses before touching the device, and leaves the output alone
+    rng = np.random.default_rng(4)
+    x = rng.random((5, 3))
+    out = np.ones((4, kt))
+    with pytest.raises(S.Unsupported, match="mixed precision is not supported"):
+        S.wall_evaluate(kernel, x, rng.random((5, ks)), rng.random((4, 3)), out, precision=S.PREC.MIXED)
+    assert np.all(out == 1.0)
+    with pytest.raises(S.KernelNotFound):
+        S.wall_evaluate(S.KERNEL.PVel, x, rng.random((5, 4)), rng.random((4, 3)), np.zeros((4, 4)))
+
+
 def test_stk3dfmm_inactive_kernel():
     fmm = S.Stk3DFMM(10, 2000, S.PAXIS.NONE, S.KERNEL.PVel | S.KERNEL.LapPGrad)
     assert fmm.is_kernel_active(S.KERNEL.PVel) and not fmm.is_kernel_active(S.KERNEL.RPY)
