@@ -131,6 +131,19 @@ int stkfmm_p2p_eval_device(unsigned kernel, unsigned p2p, int precision, int nSr
                            const double *d_srcValue, int nTrg, const double *d_trgCoord, double *d_trgValue,
                            void *stream);
 
+/* No counterpart in the reference: stkfmm_p2p_eval_device for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, semantics and kernels; it calls no hipMalloc/hipFree.  Its work list and partial sums
+ * live in a fixed 64 MiB device array per GPU, held from the library's load until the process ends,
+ * so a sum whose targets alone need more (about 2M targets at 4 output values, 512K at 16) returns
+ * STKFMM_ERR_ARG.  While the plan-based entry's partial sums fit the array the results are bitwise
+ * identical to it; above that the sources are cut into fewer chunks (same pairs, other grouping).
+ * Every call on one GPU uses that GPU's array: calls that overlap on one device (different streams,
+ * or several host threads without ordering) race and corrupt each other's results. */
+int stkfmm_p2p_eval_device_noalloc(unsigned kernel, unsigned p2p, int precision, int nSrc,
+                                   const double *d_srcCoord, const double *d_srcValue, int nTrg,
+                                   const double *d_trgCoord, double *d_trgValue, void *stream);
+
 /* ---- boundary 2: pvfmm::Kernel<double>::Ker_t micro-kernel entries ------------------------- */
 
 /* Each has the Ker_t signature (r_src, src_cnt, v_src, dof, r_trg, trg_cnt, v_trg, mem_mgr) whose

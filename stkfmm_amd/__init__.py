@@ -389,8 +389,12 @@ def _oarr(t):
     return StkOutArray(t.data_ptr(), t.stride(0), t.stride(1))
 
 
-def p2p_device(kernel, p2p, src_coord, src_value, trg_coord, trg_value, precision=PREC.FP64, stream=None):
-    """Direct sum on device tensors (contiguous AoS [n,3], [n,ks], [nt,3], [nt,kt]); trg_value +=."""
+def p2p_device(kernel, p2p, src_coord, src_value, trg_coord, trg_value, precision=PREC.FP64, stream=None,
+               no_alloc=False):
+    """Direct sum on device tensors (contiguous AoS [n,3], [n,ks], [nt,3], [nt,kt]); trg_value +=.
+
+    no_alloc=True calls the entry that allocates no device memory (its work list and partial sums sit
+    in a fixed per-device array; sizes above it raise ArgumentError)."""
     uk = get_uker(kernel, p2p)
     ks, kt = uker_dims(uk)
     ns, nt = src_coord.shape[0], trg_coord.shape[0]
@@ -399,9 +403,9 @@ def p2p_device(kernel, p2p, src_coord, src_value, trg_coord, trg_value, precisio
         _dev2d(t, n, k, nm)
         if not t.is_contiguous() or t.shape[1] != k or t.shape[0] != n:
             raise ArgumentError(f"{nm}: expected a contiguous ({n}, {k}) tensor")
-    _check(lib().stkfmm_p2p_eval_device(int(kernel), int(p2p), int(precision), ns, src_coord.data_ptr(),
-                                        src_value.data_ptr(), nt, trg_coord.data_ptr(), trg_value.data_ptr(),
-                                        _stream_handle(stream)))
+    entry = lib().stkfmm_p2p_eval_device_noalloc if no_alloc else lib().stkfmm_p2p_eval_device
+    _check(entry(int(kernel), int(p2p), int(precision), ns, src_coord.data_ptr(), src_value.data_ptr(), nt,
+                 trg_coord.data_ptr(), trg_value.data_ptr(), _stream_handle(stream)))
     return trg_value
 
 

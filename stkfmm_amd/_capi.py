@@ -50,6 +50,7 @@ EXPORTS = [
     "stkfmm_wall_eval_device", "stkfmm_scale_coord", "stkfmm_scale_src", "stkfmm_scale_trg",
     "stkfmm_scale_coord_device", "stkfmm_scale_src_device", "stkfmm_scale_trg_device", "stkfmm_tree_node_info",
     "stkfmm_tree_get_nodes", "stkfmm_tree_get_wx", "stkfmm_get_s2l_uker", "stkfmm_get_m2t_uker",
+    "stkfmm_p2p_eval_device_noalloc",
 ] + ["stkfmm_ker_" + n for n in KER_T_NAMES]
 
 
@@ -64,6 +65,8 @@ def _declare(L):
     L.stkfmm_p2p_eval.argtypes = [c_uint, c_uint, c_int, c_int, dptr, dptr, c_int, dptr, dptr]
     L.stkfmm_p2p_eval_device.argtypes = [c_uint, c_uint, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p,
                                          c_void_p, c_void_p]
+    L.stkfmm_p2p_eval_device_noalloc.argtypes = [c_uint, c_uint, c_int, c_int, c_void_p, c_void_p, c_int,
+                                                 c_void_p, c_void_p, c_void_p]
     L.stkfmm_ker_eval.argtypes = [c_int, dptr, c_int, dptr, c_int, dptr, c_int, dptr]
     for n in KER_T_NAMES:
         f = getattr(L, "stkfmm_ker_" + n)

@@ -377,10 +377,7 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
     if (max_src <= 0 && persistent == 3) {
         max_src = 512; // per-group launches: split like a stand-alone direct sum of that group
     } else if (max_src <= 0) {
-        // about 16 items per concurrently resident wave keeps the tail of the cost-sorted queue short
-        const long long target_items = 16LL * concurrent;
-        max_src = (total_cost + target_items - 1) / std::max(1LL, target_items);
-        max_src = std::max(512LL, ((max_src + 511) / 512) * 512);
+        max_src = default_max_src(total_cost, concurrent);
     }
     if (max_src < 1)
         max_src = 1;
@@ -408,10 +405,7 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
             tl.trg_count = std::min(TT, e - tb);
             tl.slot_begin = (int)items.size();
             // lanes per target: the largest power of two (at most 16) that still fits the tile
-            int lsplit = 0;
-            if (tn.split_tiles >= 0)
-                while (lsplit < 4 && (tl.trg_count << (lsplit + 1)) <= TT)
-                    ++lsplit;
+            const int lsplit = (tn.split_tiles >= 0) ? tile_lsplit(tl.trg_count, TT) : 0;
             Item cur{tl.trg_begin, tl.trg_count, (int)ranges.size(), (int)ranges.size(), 0, level, 0, lsplit};
             long long cur_n = 0;
             for (int r = r0; r < r1; ++r) {
@@ -777,8 +771,8 @@ int direct_plan(int uk, int prec, int n_src, int n_trg, stkfmm_plan **out) {
     // 3.91 -> 3.77 ms with scalar broadcast (mixed: no tuning measurably better).  Only the
     // target-to-lane mapping and the staging change, not the order in which a target sums its sources.
     stkfmm_tune tn{};
-    const bool small = n_trg < 32768;
-    tn.targets_per_lane = small ? 1 : 2;
+    const bool small = direct_targets_per_lane(n_trg) == 1;
+    tn.targets_per_lane = direct_targets_per_lane(n_trg);
     tn.broadcast = (!small && prec == STKFMM_PREC_FP64) ? 1 : 0;
     stkfmm_plan *p = nullptr;
     int rc = stkfmm_plan_create(&p, uk, prec, n_src, n_trg, 1, &gb, &ge, nullptr, gp, &rb, &re, &tn);

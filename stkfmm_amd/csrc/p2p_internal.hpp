@@ -119,6 +119,29 @@ struct UkerOps {
 constexpr int kNumCoreUker = 14;   // micro-kernels 0..13 (p2p_kernels.hip)
 constexpr int kNumUkerTotal = 25;  // 14..24 in p2p_family.hip
 
+// Sources per work item when the caller does not choose: about 16 items per concurrently resident
+// wave keeps the tail of the cost-sorted queue short; total_cost = sum over target tiles of the
+// tile's source count, concurrent = resident waves of the item kernel on the device.
+inline long long default_max_src(long long total_cost, long long concurrent) {
+    const long long target_items = 16LL * concurrent;
+    long long m = (total_cost + target_items - 1) / (target_items > 1 ? target_items : 1);
+    m = ((m + 511) / 512) * 512;
+    return m > 512 ? m : 512;
+}
+
+// Targets per lane of a single-group direct sum: one below 32,768 targets (twice the waves for a
+// small sum), else two.
+inline int direct_targets_per_lane(int n_trg) { return n_trg < 32768 ? 1 : 2; }
+
+// lanes per target of a tile of n targets (log2): the largest power of two, at most 16, that still
+// fits the tile's 64*T lanes
+__host__ __device__ inline int tile_lsplit(int n, int TT) {
+    int l = 0;
+    while (l < 4 && (n << (l + 1)) <= TT)
+        ++l;
+    return l;
+}
+
 // table row of micro-kernel uk, or nullptr
 const UkerOps *uker_ops(int uk);
 const UkerOps *uker_ops_family(int uk);
