@@ -37,7 +37,11 @@ sys.path.insert(0, ROOT)
 PEAK_FP64 = 78.6e12   # MI355X vector FP64, FLOP/s
 PEAK_FP32 = 157.3e12  # MI355X vector FP32 (published, packed), FLOP/s
 # algorithmic FLOPs per pair and the FP64 share in mixed mode (displacement + accumulation)
-FLOPS = {"pvel_fused": (72, 7), "pvel_sl": (35, 7), "rpy": (56, 9), "stokeslet": (29, 6), "lapgrad": (20, 7)}
+FLOPS = {"pvel_fused": (72, 7), "pvel_sl": (35, 7), "rpy": (56, 9), "stokeslet": (29, 6), "lapgrad": (20, 7),
+         # adjoint of stokes_pvel per pair: displacement 3, r^2 5, rsqrt 1, powers of 1/r 3, f.r 5, a - T 1,
+         # w.r 5, w.f 5, the two coefficients 3 + 8, (a - T)/r^3 1, position gradient 15, value gradient
+         # 12 + 2, target and source position sums 3 + 3; mixed keeps only the displacement in FP64
+         "pvel_vjp": (75, 3)}
 NOTE = ("VALU-bound; peak = MI355X published vector FP64 78.6 / FP32 157.3 TFLOP/s combined by the "
         "FP64/FP32 split of the algorithmic FLOPs; the library is compiled with packed FP32 disabled")
 
@@ -53,7 +57,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU work budget of the baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-per-leaf", action="store_true")
-    ap.add_argument("--only", choices=["mixed", "fp64", "per-leaf"], default=None,
+    ap.add_argument("--no-target-4m", action="store_true",
+                    help="default workload: skip the compact 4M-point batched-versus-per-leaf measurement")
+    ap.add_argument("--only", choices=["mixed", "fp64", "per-leaf", "mixed-sorted", "fp64-sorted"], default=None,
                     help="run one variant only (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device copy of known size (PMC calibration) and exit")
@@ -174,7 +180,55 @@ def reference_sample(O, uker, tree, xs, allv, kt, leaf_order, budget_s, nth):
     return info, np.concatenate(rows), np.concatenate(outs)
 
 
-def ulist_bench(args, torch, S, kind, uker, points, cap, values_fn, kt, metric, workload_name, variants_order):
+def target_4m(args, torch, S, W):
+    """The batched-versus-per-leaf target at 4M points (the default workload's distribution at
+    4,000,000 points, leaf cap 1000), compact enough to ride along the default run: mixed and FP64
+    batched launches with the shipped tuning, the plain FP64 per-leaf launch, and each one's error
+    against the oracle on a random sample of whole leaves."""
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    x = W.pvel_points(4_000_000)
+    tree = S.Tree(x, 1000)
+    xs = x[tree.perm]
+    sl, dl = W.pvel_values(tree.n)
+    setup_s = time.perf_counter() - t0
+    d_x, d_sl, d_dl = (torch.from_numpy(a).to(dev) for a in (xs, sl, dl))
+    out = torch.zeros((tree.n, 4), dtype=torch.float64, device=dev)
+    kw = dict(src_coord=d_x, src_value=d_sl, trg_coord=d_x, trg_value=out, src_value2=d_dl)
+    stream = torch.cuda.Stream()
+    uk = S.UKER.STOKES_PVEL_FUSED
+    res, outs = {}, {}
+    for name, prec, tune, steps in (("mixed", S.PREC.MIXED, "auto", 5), ("fp64", S.PREC.FP64, "auto", 3),
+                                    ("per-leaf", S.PREC.FP64, dict(persistent=3), 1)):
+        plan = S.Plan.from_tree(tree, uk, prec, tune=tune)
+        wall, item_ms, _, _ = Timed(torch, plan, kw, stream)(steps, 1)
+        r = torch.zeros_like(out)
+        plan.execute(stream=stream, **dict(kw, trg_value=r))
+        torch.cuda.synchronize()
+        outs[name] = r.cpu().numpy()
+        res[name] = {"gpair_s": tree.pairs * steps / wall / 1e9, "ms_per_step": 1e3 * wall / steps,
+                     "item_ms": item_ms}
+        del plan, r
+        print(f"# target 4M {name}: {res[name]['gpair_s']:.1f} Gpair/s", file=sys.stderr, flush=True)
+    acc = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        sel = np.random.default_rng(4).choice(tree.n_leaves, 48, replace=False)
+        ref = O.ranges(uk, xs, np.concatenate([sl, dl], axis=1), xs, tree.leaf_begin, tree.leaf_end, tree.ulist_ptr,
+                       tree.rng_begin, tree.rng_end, sel=sel, nthreads=threads(), simd=True)
+        rows = np.concatenate([np.arange(tree.leaf_begin[l], tree.leaf_end[l]) for l in sel])
+        acc = {}
+        for name, o in outs.items():
+            l2, mx = O.max_error(o[rows], ref[rows])
+            acc[name] = {"errorL2": l2, "errorMaxRel": mx, "targets_checked": int(rows.size), "leaves_checked": 48}
+    return {"workload": "stokes-pvel-ulist-4M-clustered-leafcap1000", "points": tree.n, "leaves": tree.n_leaves,
+            "pairs": tree.pairs, "setup_s": setup_s, "variants": res, "accuracy_vs_oracle": acc,
+            "speedup_mixed_vs_fp64_per_leaf_launch": res["mixed"]["gpair_s"] / res["per-leaf"]["gpair_s"],
+            "target": ">= 10x the FP64 per-leaf launch at <= 1e-6 relative error"}
+
+
+def ulist_bench(args, torch, S, kind, uker, points, cap, values_fn, kt, metric, workload_name, variants_order,
+                extra_fields=None):
     from stkfmm_amd.autotune import autotune
 
     dev = torch.device("cuda:0")
@@ -291,6 +345,8 @@ def ulist_bench(args, torch, S, kind, uker, points, cap, values_fn, kt, metric, 
                 ref_acc[name] = {"errorL2": l2, "errorMaxRel": mx, "targets_checked": int(ref_rows.size)}
 
     extra = dict(variants=variants, accuracy_vs_oracle=accuracy, roofline=rl, cpu_baseline=cpu)
+    if extra_fields:
+        extra.update(extra_fields)
     if not args.no_cpu_baseline and ref_cpu is not None:
         extra.update(cpu_reference=ref_cpu, accuracy_vs_reference=ref_acc)
     if "per-leaf" in variants and head == "mixed":
@@ -380,19 +436,27 @@ def pvel_adjoint(args, torch, S, W):
         S.uker_device(uk, d_src, d_val, d_trg, out)
     torch.cuda.synchronize()
     fwd = (time.perf_counter() - t) / args.steps
+    # HIP events on torch's current stream, the stream the library launches the adjoint on
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t = time.perf_counter()
+    ev0.record()
     for _ in range(args.steps):
         S.vjp_device(uk, d_src, d_val, d_trg, d_w)
+    ev1.record()
     torch.cuda.synchronize()
     adj = (time.perf_counter() - t) / args.steps
+    adj_kernel_ms = ev0.elapsed_time(ev1) / args.steps
     for _ in range(args.warmup):
         S.vjp_device(uk, d_src, d_val, d_trg, d_w, precision=S.PREC.MIXED)
     torch.cuda.synchronize()
     t = time.perf_counter()
+    ev0.record()
     for _ in range(args.steps):
         S.vjp_device(uk, d_src, d_val, d_trg, d_w, precision=S.PREC.MIXED)
+    ev1.record()
     torch.cuda.synchronize()
     adj_mixed = (time.perf_counter() - t) / args.steps
+    adj_mixed_kernel_ms = ev0.elapsed_time(ev1) / args.steps
     a = d_src.clone().requires_grad_(True)
     b = d_val.clone().requires_grad_(True)
     c = d_trg.clone().requires_grad_(True)
@@ -431,6 +495,11 @@ def pvel_adjoint(args, torch, S, W):
          forward_ms=1e3 * fwd, adjoint_ms=1e3 * adj, autograd_fwd_bwd_ms=1e3 * ag,
          forward_gpair_s=pairs / fwd / 1e9, adjoint_gpair_s=pairs / adj / 1e9,
          adjoint_mixed_ms=1e3 * adj_mixed, adjoint_mixed_gpair_s=pairs / adj_mixed / 1e9,
+         roofline=dict(roofline("pvel_vjp", "fp64", pairs, adj_kernel_ms, "vjp_pair_kernel<stokes_pvel,FP64,T=2>"),
+                       note=NOTE + "; kernel_ms = the adjoint's kernels (pair pass + partial-sum reductions) "
+                                   "between HIP events on their stream"),
+         roofline_mixed=roofline("pvel_vjp", "mixed", pairs, adj_mixed_kernel_ms,
+                                 "vjp_pair_kernel<stokes_pvel,MIXED,T=2>"),
          cpu_baseline=cpu, accuracy_vs_oracle=acc)
 
 
@@ -606,20 +675,29 @@ def wall_images(args, torch, S, W):
 
 
 def stokeslet_instances(args, torch, S, W):
+    """512 independent Stokeslet direct sums (src = trg per instance) in one HIP-graph replay.  Variants:
+    each precision with the points as given ("unsorted") and with the plan ordering each instance's
+    points along a Morton curve inside the same graph ("sorted", stkfmm_tune.sort_points; the ordering
+    kernels are part of every replay and of the reported latency)."""
     dev = torch.device("cuda:0")
     sizes, offsets, coord, f = W.stokeslet_instances()
     n = int(offsets[-1])
     pairs = int(np.sum(sizes.astype(np.int64) ** 2))
     d_c, d_f = torch.from_numpy(coord).to(dev), torch.from_numpy(f).to(dev)
-    res = {}
-    for prec in (S.PREC.MIXED, S.PREC.FP64):
-        gb, ge = offsets[:-1], offsets[1:]
-        # FP64: scalar-broadcast staging (measured 61.7 -> 59.8 ms item kernel; mixed keeps the defaults,
-        # best among T = 1, 2, 4, broadcast and 1024 / 4096 sources per item)
-        tune = dict(broadcast=1) if prec == S.PREC.FP64 else None
+    gb, ge = offsets[:-1], offsets[1:]
+    res, results = {}, {}
+    # FP64 unsorted: scalar-broadcast staging (measured 61.7 -> 59.8 ms item kernel)
+    variants = [("mixed-sorted", S.PREC.MIXED, dict(sort_points=1)), ("fp64-sorted", S.PREC.FP64, dict(sort_points=1)),
+                ("mixed", S.PREC.MIXED, None), ("fp64", S.PREC.FP64, dict(broadcast=1))]
+    if args.only:
+        variants = [v for v in variants if v[0] == args.only]
+    stream = torch.cuda.Stream()
+    for name, prec, tune in variants:
         plan = S.Plan(S.UKER.STOKES_VEL, prec, n, n, gb, ge, None, np.arange(len(sizes) + 1), gb, ge, tune=tune)
         out = torch.zeros((n, 3), dtype=torch.float64, device=dev)
-        stream = torch.cuda.Stream()
+        # item-kernel time of eager executes (HIP events on the execution stream)
+        _, item_ms, reduce_ms, _ = Timed(torch, plan, dict(src_coord=d_c, src_value=d_f, trg_coord=d_c,
+                                                          trg_value=out), stream)(3, 1)
         torch.cuda.synchronize()
         plan.capture_graph(d_c, d_f, d_c, out, stream=stream)
         for _ in range(args.warmup):
@@ -631,14 +709,56 @@ def stokeslet_instances(args, torch, S, W):
             plan.launch_graph(stream=stream)
             stream.synchronize()
             lat.append(time.perf_counter() - t)
-        res[prec.name.lower()] = dict(gpair_s=pairs / float(np.mean(lat)) / 1e9,
-                                      batch_latency_ms=1e3 * float(np.median(lat)),
-                                      batch_latency_min_ms=1e3 * float(np.min(lat)), n_items=plan.stats()["n_items"])
-    m = res["mixed"]
-    line("Gpair-interactions/s 512 Stokeslet direct sums in one HIP-graph replay (aggregate)", m["gpair_s"],
-         "Gpair/s", args, m["batch_latency_ms"], "f64+f32 (mixed)",
+        o = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+        plan.execute(d_c, d_f, d_c, o, stream=stream)
+        stream.synchronize()
+        results[name] = o.cpu().numpy()
+        res[name] = dict(gpair_s=pairs / float(np.mean(lat)) / 1e9, batch_latency_ms=1e3 * float(np.median(lat)),
+                         batch_latency_min_ms=1e3 * float(np.min(lat)), item_ms=item_ms, reduce_ms=reduce_ms,
+                         n_items=plan.stats()["n_items"], tune=tune)
+        print(f"# stokeslet-instances {name}: {res[name]['gpair_s']:.1f} Gpair/s, batch "
+              f"{res[name]['batch_latency_ms']:.3f} ms, item kernel {item_ms:.3f} ms", file=sys.stderr, flush=True)
+    head = next(iter(res))
+    h = res[head]
+    prec_name = "fp64" if head.startswith("fp64") else "mixed"
+    kname = {"mixed-sorted": "p2p_items_kernel<6,MIXED,T=2,far-tile form>", "mixed": "p2p_items_kernel<6,MIXED,T=2>",
+             "fp64-sorted": "p2p_items_kernel<6,FP64,T=2>", "fp64": "p2p_items_sgpr_kernel<6,FP64,T=2>"}[head]
+    rl = roofline("stokeslet", prec_name, pairs, h["item_ms"], kname)
+    rl["note"] = NOTE + "; kernel_ms = item kernel of an eager execute of the same plan (the replay adds the ordering)"
+    cpu = acc = None
+    if not args.no_cpu_baseline:
+        # the oracle on whole instances in random order until the CPU budget is spent: timed as the CPU
+        # baseline and used as the checker of every variant on those instances
+        from oracle import oracle as O
+        nth = threads()
+        order = np.random.default_rng(55).permutation(len(sizes))
+        done, cpu_pairs, cpu_s, refs = [], 0, 0.0, {}
+        for i in order:
+            a, b = int(offsets[i]), int(offsets[i + 1])
+            t = time.perf_counter()
+            refs[i] = O.direct(6, coord[a:b], f[a:b], coord[a:b], nthreads=nth)
+            cpu_s += time.perf_counter() - t
+            cpu_pairs += (b - a) ** 2
+            done.append(i)
+            if cpu_s >= args.cpu_seconds:
+                break
+        rows = np.concatenate([np.arange(offsets[i], offsets[i + 1]) for i in done])
+        ref = np.concatenate([refs[i] for i in done])
+        acc = {}
+        for name, r in results.items():
+            l2, mx = O.max_error(r[rows], ref)
+            acc[name] = {"errorL2": l2, "errorMaxRel": mx, "targets_checked": int(rows.size)}
+        cpu = {"value": cpu_pairs / cpu_s / 1e9, "unit": "Gpair/s", "cores": nth, "kind": "port",
+               "sample": (f"{len(done)} of {len(sizes)} whole instances in random order, {rows.size} targets, "
+                          f"{cpu_pairs:.3e} pairs, {cpu_s:.1f} s; the oracle's scalar FP64 restatement of the "
+                          "Stokeslet (PVFMM stokes_vel closed form), OpenMP over targets"),
+               **host_info()}
+    line("Gpair-interactions/s 512 Stokeslet direct sums in one HIP-graph replay (aggregate)", h["gpair_s"],
+         "Gpair/s", args, h["batch_latency_ms"], "f64+f32 (mixed)" if prec_name == "mixed" else "f64",
          {"workload": "stokeslet-512-instances", "instances": len(sizes), "points": n, "pairs": pairs,
-          "precision": "mixed"}, variants=res)
+          "precision": prec_name, "variant": head,
+          "points_order": "sorted on the device inside the graph" if head.endswith("sorted") else "as given"},
+         variants=res, roofline=rl, cpu_baseline=cpu, accuracy_vs_oracle=acc)
 
 
 def main():
@@ -663,9 +783,11 @@ def main():
         print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8}))
         return
     if args.workload == "pvel-ulist":
+        t4m = None if args.no_target_4m else {"target_4m": target_4m(args, torch, S, W)}
+        torch.cuda.empty_cache()
         ulist_bench(args, torch, S, "pvel_fused", S.UKER.STOKES_PVEL_FUSED, W.pvel_points, 1000, W.pvel_values, 4,
                     "Gpair-interactions/s StokesPVel P2P: FP64 vs mixed at ≤1e-6 rel err",
-                    "stokes-pvel-ulist-1M-clustered-leafcap1000", ["mixed", "fp64", "per-leaf"])
+                    "stokes-pvel-ulist-1M-clustered-leafcap1000", ["mixed", "fp64", "per-leaf"], extra_fields=t4m)
     elif args.workload == "pvel-ulist-4m":
         # the same point distribution at 4M points (the size of the batched-vs-per-leaf target)
         ulist_bench(args, torch, S, "pvel_fused", S.UKER.STOKES_PVEL_FUSED, lambda: W.pvel_points(4_000_000), 1000,

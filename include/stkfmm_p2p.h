@@ -136,8 +136,10 @@ int stkfmm_p2p_eval_device(unsigned kernel, unsigned p2p, int precision, int nSr
  * arguments, semantics and kernels; it calls no hipMalloc/hipFree.  Its work list and partial sums
  * live in a fixed 64 MiB device array per GPU, held from the library's load until the process ends,
  * so a sum whose targets alone need more (about 2M targets at 4 output values, 512K at 16) returns
- * STKFMM_ERR_ARG.  While the plan-based entry's partial sums fit the array the results are bitwise
- * identical to it; above that the sources are cut into fewer chunks (same pairs, other grouping).
+ * STKFMM_ERR_ARG.  Below 16,384 sources or targets, and while the plan-based entry's partial sums fit
+ * the array, the results are bitwise identical to it; otherwise the same pairs are summed in another
+ * grouping (the plan-based entry orders larger sums' points along a Morton curve; this one keeps the
+ * caller's order, and cuts the sources into fewer chunks when its array is the limit).
  * Every call on one GPU uses that GPU's array: calls that overlap on one device (different streams,
  * or several host threads without ordering) race and corrupt each other's results. */
 int stkfmm_p2p_eval_device_noalloc(unsigned kernel, unsigned p2p, int precision, int nSrc,
@@ -238,6 +240,14 @@ typedef struct stkfmm_tune {
                              sources stay in that XCD's L2; a wave drains its own queue, then steals
                              from the others.  Measured on the 1M PVel U-list: 22% fewer L2 fetches,
                              1.5% longer (the kernel is VALU-bound) */
+    int sort_points;      /* 1: at every execute, order each group's targets and each group's source
+                             range along a Morton curve of their own bounding box (on the device, into
+                             plan-owned copies; results still go to the original target rows), so that
+                             target tiles are compact and most source tiles skip the coincidence mask
+                             (mixed mode: and form the displacement in FP32, the far-tile form).  For
+                             work lists of unsorted points (direct sums, independent instances); needs
+                             exactly one source range per group, pairwise disjoint target ranges and
+                             source ranges that are pairwise disjoint or identical.  0: as given */
 } stkfmm_tune;
 
 typedef struct stkfmm_plan stkfmm_plan;

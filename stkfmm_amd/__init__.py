@@ -689,7 +689,7 @@ class Plan:
             t = StkTune(int(tune.get("targets_per_lane", 0)), int(tune.get("max_src_per_item", 0)),
                         int(tune.get("persistent", 0)), int(tune.get("waves_per_cu", 0)),
                         int(tune.get("split_tiles", 0)), int(tune.get("broadcast", 0)),
-                        int(tune.get("queues", 0)))
+                        int(tune.get("queues", 0)), int(tune.get("sort_points", 0)))
         h = ctypes.c_void_p()
         _check(lib().stkfmm_plan_create(ctypes.byref(h), int(uker), int(precision), int(n_src), int(n_trg), ng,
                                         ip(gb), ip(ge), ip(gl), ip(gp), ip(rb), ip(re),

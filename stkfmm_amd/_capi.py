@@ -27,7 +27,8 @@ class StkOutArray(ctypes.Structure):
 
 class StkTune(ctypes.Structure):
     _fields_ = [("targets_per_lane", c_int), ("max_src_per_item", c_int), ("persistent", c_int),
-                ("waves_per_cu", c_int), ("split_tiles", c_int), ("broadcast", c_int), ("queues", c_int)]
+                ("waves_per_cu", c_int), ("split_tiles", c_int), ("broadcast", c_int), ("queues", c_int),
+                ("sort_points", c_int)]
 
 
 KER_T_NAMES = ["laplace_p", "laplace_dipolep", "laplace_pgrad", "laplace_dipolepgrad", "laplace_pgradgrad",

@@ -5,10 +5,12 @@
 // of the host uploading it, and everything the item and reduce kernels need beyond the caller's
 // inputs and output -- items, source ranges, tiles, the queue bounds and counter, the per-item
 // partial sums -- lives in one file-scope device array, `scratch`, of kScratchBytes per device.
-// The kernels are the ones the cached-plan entry (stkfmm_p2p_eval_device) runs, with the same
-// tiling: while the partial sums fit, the chunking is the plan's as well and the two entries give
-// bitwise identical results; above that the sources are cut into fewer, longer chunks (the same
-// pairs, summed in a different grouping).
+// The kernels are the ones the cached-plan entry (stkfmm_p2p_eval_device) runs on points in the
+// caller's order: below 16,384 sources or targets (where that entry does not order the points along a
+// Morton curve) and while the partial sums fit, the tiling and chunking are the plan's as well and the
+// two entries give bitwise identical results; otherwise the same pairs are summed in another grouping
+// (no point ordering here, which would need copies of the points; above the plan's partial-sum size,
+// fewer and longer source chunks).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>

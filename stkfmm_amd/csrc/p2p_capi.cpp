@@ -273,6 +273,18 @@ struct stkfmm_plan {
     int kernel_prec = 0;
     int level_min = 0, n_copies = 0, rec_words = 0;
     double *d_recs = nullptr;
+    // sort_points: Morton ordering of each execute's points (morton.hip) into the SoA copies below
+    bool sorted = false;
+    int n_sort_segs = 0, n_sort_blks = 0;
+    // every group's target range is its source range: with one coordinate array for both, only the
+    // source segments (the first n_src_segs segments, n_src_blks blocks) need ordering
+    bool self_ranges = false;
+    int n_src_segs = 0, n_src_blks = 0;
+    SortSeg *d_sort_segs = nullptr;
+    SortBlk *d_sort_blks = nullptr;
+    double *d_sort_box = nullptr, *d_src_coord = nullptr, *d_src_value = nullptr, *d_trg_coord = nullptr;
+    int *d_sort_hist = nullptr, *d_src_idx = nullptr, *d_trg_idx = nullptr, *d_trg_row = nullptr;
+    unsigned short *d_sort_keys = nullptr;
 };
 
 static void plan_free(stkfmm_plan *p) {
@@ -291,6 +303,17 @@ static void plan_free(stkfmm_plan *p) {
     (void)hipFree(p->d_counter);
     (void)hipFree(p->d_qptr);
     (void)hipFree(p->d_recs);
+    (void)hipFree(p->d_sort_segs);
+    (void)hipFree(p->d_sort_blks);
+    (void)hipFree(p->d_sort_box);
+    (void)hipFree(p->d_src_coord);
+    (void)hipFree(p->d_src_value);
+    (void)hipFree(p->d_trg_coord);
+    (void)hipFree(p->d_sort_hist);
+    (void)hipFree(p->d_src_idx);
+    (void)hipFree(p->d_trg_idx);
+    (void)hipFree(p->d_trg_row);
+    (void)hipFree(p->d_sort_keys);
     delete p;
 }
 
@@ -320,7 +343,7 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
     if (n_groups > 0 && (!grp_trg_begin || !grp_trg_end || !grp_rng_ptr))
         return set_error(STKFMM_ERR_ARG, "group arrays are NULL");
 
-    stkfmm_tune tn{0, 0, 0, 0, 0, 0};
+    stkfmm_tune tn{};
     if (tune)
         tn = *tune;
     const int T = tn.targets_per_lane ? tn.targets_per_lane : 2;
@@ -332,6 +355,8 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
     const int broadcast = tn.broadcast;
     if (broadcast != 0 && broadcast != 1)
         return set_error(STKFMM_ERR_ARG, "broadcast must be 0 (LDS) or 1 (scalar) (got %d)", broadcast);
+    if (tn.sort_points != 0 && tn.sort_points != 1)
+        return set_error(STKFMM_ERR_ARG, "sort_points must be 0 or 1 (got %d)", tn.sort_points);
 
     // validate the work list
     long long total_cost = 0; // sum over tiles of the tile's source count
@@ -357,14 +382,64 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
         total_cost += ns * (long long)((e - b + TT - 1) / TT);
     }
 
+    // Morton ordering: segments (distinct source ranges, group target ranges) and their blocks
+    std::vector<SortSeg> segs;
+    std::vector<SortBlk> blks;
+    long long hist_len = 0;
+    bool self_ranges = false;
+    int n_src_segs = 0, n_src_blks = 0;
+    if (tn.sort_points) {
+        std::vector<std::pair<int, int>> tr, sr;
+        self_ranges = true;
+        for (int g = 0; g < n_groups; ++g) {
+            const int r0 = grp_rng_ptr[g], r1 = grp_rng_ptr[g + 1];
+            if (grp_trg_end[g] == grp_trg_begin[g] || r1 == r0)
+                continue;
+            if (r1 - r0 != 1)
+                return set_error(STKFMM_ERR_UNSUPPORTED, "sort_points: group %d has %d source ranges (needs one)", g,
+                                 r1 - r0);
+            tr.emplace_back(grp_trg_begin[g], grp_trg_end[g]);
+            if (rng_src_end[r0] > rng_src_begin[r0])
+                sr.emplace_back(rng_src_begin[r0], rng_src_end[r0]);
+            self_ranges = self_ranges && rng_src_begin[r0] == grp_trg_begin[g] && rng_src_end[r0] == grp_trg_end[g];
+        }
+        std::sort(tr.begin(), tr.end());
+        std::sort(sr.begin(), sr.end());
+        sr.erase(std::unique(sr.begin(), sr.end()), sr.end());
+        for (size_t i = 1; i < tr.size(); ++i)
+            if (tr[i].first < tr[i - 1].second)
+                return set_error(STKFMM_ERR_UNSUPPORTED, "sort_points: target ranges of two groups overlap");
+        for (size_t i = 1; i < sr.size(); ++i)
+            if (sr[i].first < sr[i - 1].second)
+                return set_error(STKFMM_ERR_UNSUPPORTED,
+                                 "sort_points: two source ranges overlap without being identical");
+        auto add = [&](const std::vector<std::pair<int, int>> &v, int kind) {
+            for (const auto &r : v) {
+                SortSeg sg{r.first, r.second, (int)blks.size(), 0, sort_level(r.second - r.first), kind};
+                for (int b = r.first; b < r.second; b += kSortBlock) {
+                    blks.push_back(SortBlk{(int)segs.size(), b, std::min(r.second, b + kSortBlock), hist_len});
+                    hist_len += 1LL << (3 * sg.level);
+                }
+                sg.blk_end = (int)blks.size();
+                segs.push_back(sg);
+            }
+        };
+        add(sr, 0);
+        n_src_segs = (int)segs.size();
+        n_src_blks = (int)blks.size();
+        add(tr, 1);
+    }
+
     int n_cu = 0, bpc = 0;
     int rc = device_cu_count(&n_cu);
     if (rc)
         return rc;
     // the far-tile form pays where targets and sources are spatially sorted (groups with octree
-    // levels); for direct sums of unsorted points its extra registers only cost occupancy
-    const int kernel_prec = (precision == STKFMM_PREC_MIXED && grp_level && T == 2 && !broadcast) ? kPrecMixedFar
-                                                                                                 : precision;
+    // levels, or points the plan orders itself); for direct sums of unsorted points its extra
+    // registers only cost occupancy
+    const bool coherent = grp_level || !segs.empty();
+    const int kernel_prec = (precision == STKFMM_PREC_MIXED && coherent && T == 2 && !broadcast) ? kPrecMixedFar
+                                                                                                  : precision;
     if (broadcast)
         HIP_TRY(items_sgpr_occupancy(uk, precision, T, &bpc), "occupancy query");
     else
@@ -541,9 +616,41 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
         plan_free(p);
         return r;
     }
+    if (!segs.empty()) {
+        p->sorted = true;
+        p->n_sort_segs = (int)segs.size();
+        p->n_sort_blks = (int)blks.size();
+        p->self_ranges = self_ranges;
+        p->n_src_segs = n_src_segs;
+        p->n_src_blks = n_src_blks;
+        if ((r = upload(&p->d_sort_segs, segs, "sort segments")) || (r = upload(&p->d_sort_blks, blks, "sort blocks"))) {
+            plan_free(p);
+            return r;
+        }
+    }
     const size_t partial_elems = (size_t)p->n_items * p->KT * p->TT;
     hipError_t e = hipSuccess;
-    if (partial_elems)
+    if (p->sorted) {
+        const size_t ns = (size_t)n_src, nt = (size_t)n_trg;
+        e = hipMalloc((void **)&p->d_sort_box, sizeof(double) * 6 * blks.size());
+        if (e == hipSuccess)
+            e = hipMalloc((void **)&p->d_sort_hist, sizeof(int) * (size_t)hist_len);
+        if (e == hipSuccess)
+            e = hipMalloc((void **)&p->d_sort_keys, sizeof(unsigned short) * std::max<size_t>(ns + nt, 1));
+        if (e == hipSuccess)
+            e = hipMalloc((void **)&p->d_src_idx, sizeof(int) * std::max<size_t>(ns, 1));
+        if (e == hipSuccess)
+            e = hipMalloc((void **)&p->d_trg_idx, sizeof(int) * std::max<size_t>(nt, 1));
+        if (e == hipSuccess)
+            e = hipMalloc((void **)&p->d_trg_row, sizeof(int) * std::max<size_t>(nt, 1));
+        if (e == hipSuccess)
+            e = hipMalloc((void **)&p->d_src_coord, sizeof(double) * 3 * std::max<size_t>(ns, 1));
+        if (e == hipSuccess)
+            e = hipMalloc((void **)&p->d_src_value, sizeof(double) * p->KS * std::max<size_t>(ns, 1));
+        if (e == hipSuccess)
+            e = hipMalloc((void **)&p->d_trg_coord, sizeof(double) * 3 * std::max<size_t>(nt, 1));
+    }
+    if (e == hipSuccess && partial_elems)
         e = hipMalloc((void **)&p->d_partial, sizeof(double) * partial_elems);
     if (e == hipSuccess)
         e = hipMalloc((void **)&p->d_counter, sizeof(int) * (size_t)p->n_queues);
@@ -573,6 +680,41 @@ static int plan_enqueue(stkfmm_plan *p, const stkfmm_array *sc, const stkfmm_arr
     a.v1 = DArr{sv->ptr, sv->stride, sv->fstride};
     a.v2 = (p->KS1 < p->KS) ? DArr{sv2->ptr, sv2->stride, sv2->fstride} : a.v1;
     a.tc = DArr{tc->ptr, tc->stride, tc->fstride};
+    if (p->sorted) {
+        // order the points of this execute, then run on the sorted SoA copies
+        SortArgs so;
+        so.segs = p->d_sort_segs;
+        so.blks = p->d_sort_blks;
+        so.sc = a.sc;
+        so.v1 = a.v1;
+        so.v2 = a.v2;
+        so.tc = a.tc;
+        so.ks = p->KS;
+        so.ks1 = p->KS1;
+        so.n_src = p->n_src;
+        so.n_trg = p->n_trg;
+        so.blk_box = p->d_sort_box;
+        so.hist = p->d_sort_hist;
+        so.keys = p->d_sort_keys;
+        so.src_idx = p->d_src_idx;
+        so.trg_idx = p->d_trg_idx;
+        so.src_coord = p->d_src_coord;
+        so.src_value = p->d_src_value;
+        so.trg_coord = p->d_trg_coord;
+        so.trg_row = p->d_trg_row;
+        so.perm = perm;
+        // targets are the sources: same ranges, and the same coordinate array at this execute
+        so.shared = p->self_ranges && sc->ptr == tc->ptr && sc->stride == tc->stride && sc->fstride == tc->fstride;
+        HIP_TRY(launch_sort_points(so, so.shared ? p->n_src_segs : p->n_sort_segs,
+                                   so.shared ? p->n_src_blks : p->n_sort_blks, s),
+                "point ordering launch");
+        const long long ns = p->n_src, nt = p->n_trg;
+        a.sc = DArr{p->d_src_coord, 1, ns};
+        a.v1 = DArr{p->d_src_value, 1, ns};
+        a.v2 = DArr{p->d_src_value + (long long)p->KS1 * ns, 1, ns};
+        a.tc = so.shared ? a.sc : DArr{p->d_trg_coord, 1, nt};
+        perm = p->d_trg_row;
+    }
     a.items = p->d_items;
     a.ranges = p->d_ranges;
     a.n_items = p->n_items;
@@ -619,7 +761,11 @@ static int plan_enqueue(stkfmm_plan *p, const stkfmm_array *sc, const stkfmm_arr
     }
     if (timed)
         HIP_TRY(hipEventRecord(ev[1], s), "event record");
-    HIP_TRY(launch_reduce(ra, s), "p2p reduce launch");
+    // permuted rows with many tiles: one contiguous write per target (same sums, same order)
+    if (perm && p->n_tiles >= 1024)
+        HIP_TRY(launch_reduce_rows(ra, s), "p2p reduce launch");
+    else
+        HIP_TRY(launch_reduce(ra, s), "p2p reduce launch");
     if (timed)
         HIP_TRY(hipEventRecord(ev[2], s), "event record");
     return STKFMM_OK;
@@ -765,15 +911,23 @@ int direct_plan(int uk, int prec, int n_src, int n_trg, stkfmm_plan **out) {
         }
     }
     const int gb = 0, ge = n_trg, gp[2] = {0, 1}, rb = 0, re = n_src;
-    // one target group: one target per lane below 32,768 targets (twice the waves for a small sum),
-    // and in FP64 the scalar-broadcast staging at or above it.  Measured on MI355X (item kernel):
-    // 8192^2 LapPGrad 0.127 -> 0.100 ms FP64 and 0.115 -> 0.096 ms mixed at T = 1; 65,536^2 PVel FP64
-    // 3.91 -> 3.77 ms with scalar broadcast (mixed: no tuning measurably better).  Only the
-    // target-to-lane mapping and the staging change, not the order in which a target sums its sources.
+    // one target group.  From 16,384 sources and targets on, the points are ordered along a Morton
+    // curve at every call (sort_points, T = 2): measured on MI355X, whole call including the ordering,
+    // 65,536^2 PVel 3.94 -> 3.77 ms FP64 and 2.88 -> 2.70 ms mixed, 131,072^2 Stokeslet mixed 9.97 ->
+    // 8.43 ms, 16,384^2 about even, 8192^2 slower (0.11 -> 0.14 ms: the ordering's fixed cost).
+    // Below that: one target per lane under 32,768 targets (twice the waves for a small sum), and in
+    // FP64 the scalar-broadcast staging at or above it (8192^2 LapPGrad 0.127 -> 0.100 ms FP64 and
+    // 0.115 -> 0.096 ms mixed at T = 1).  Only the target-to-lane mapping and the staging change, not
+    // the order in which a target sums its sources.
     stkfmm_tune tn{};
     const bool small = direct_targets_per_lane(n_trg) == 1;
     tn.targets_per_lane = direct_targets_per_lane(n_trg);
     tn.broadcast = (!small && prec == STKFMM_PREC_FP64) ? 1 : 0;
+    if (n_trg >= kDirectSortMin && n_src >= kDirectSortMin) {
+        tn.sort_points = 1;
+        tn.targets_per_lane = 2;
+        tn.broadcast = 0;
+    }
     stkfmm_plan *p = nullptr;
     int rc = stkfmm_plan_create(&p, uk, prec, n_src, n_trg, 1, &gb, &ge, nullptr, gp, &rb, &re, &tn);
     if (rc)

@@ -89,6 +89,64 @@ struct ReduceArgs {
     double scale;         // ScaleFactor of the kernel
 };
 
+// XCD-contiguous work mapping.  Workgroups of a launch are dealt to MI355X's 8 XCDs round-robin
+// (workgroup b runs on XCD b % 8), each XCD with its own L2.  A launch of 8*ceil(n/8) workgroups that
+// takes unit xcd_unit(b, n) gives XCD x the contiguous units [x*q, (x+1)*q), q = ceil(n/8), so work on
+// neighbouring data (one instance's tiles or points) shares one L2; the result is -1 for the padding
+// workgroups past n.
+constexpr int kXcds = 8;
+__host__ __device__ inline int xcd_grid(int n) { return kXcds * ((n + kXcds - 1) / kXcds); }
+__device__ inline int xcd_unit(int b, int n) {
+    const int q = (n + kXcds - 1) / kXcds;
+    const int u = (b % kXcds) * q + b / kXcds;
+    return u < n ? u : -1;
+}
+
+// Morton ordering of a plan's points (morton.hip).  A segment is a group's target range (kind 1) or a
+// distinct source range (kind 0), ordered on a 2^level-per-axis grid over its own bounding box; a
+// block is a run of at most kSortBlock consecutive points of one segment.
+constexpr int kSortBlock = 512;
+constexpr int kSortMaxLevel = 4;
+constexpr int kSortMaxCells = 1 << (3 * kSortMaxLevel);
+struct SortSeg {
+    int begin, end;         // point range
+    int blk_begin, blk_end; // its blocks
+    int level;              // grid level, 1..kSortMaxLevel
+    int kind;               // 0 sources, 1 targets
+};
+struct SortBlk {
+    int seg;
+    int begin, end;
+    long long hist_off; // first of its 8^level histogram entries
+};
+struct SortArgs {
+    const SortSeg *segs;
+    const SortBlk *blks;
+    DArr sc, v1, v2, tc;    // the caller's arrays
+    int ks, ks1;
+    long long n_src, n_trg;
+    double *blk_box;        // [n_blks][6]
+    int *hist;              // per block, per cell: count, then first output position
+    unsigned short *keys;   // [n_src + n_trg]: cell of each point
+    int *src_idx, *trg_idx; // sorted position -> original point
+    double *src_coord;      // SoA copies in sorted order: [3][n_src]
+    double *src_value;      // [ks][n_src]
+    double *trg_coord;      // [3][n_trg]
+    int *trg_row;           // output row of sorted target p (the caller's perm applied)
+    const int *perm;        // the caller's output permutation (null: identity)
+    int shared;             // targets are the sources (same points, same ranges): the source
+                            // ordering serves both, and the source gather writes trg_row
+};
+// a segment's grid level: about 64 points per cell, at most 3 levels up to 2^18 points (finer
+// grids do not make the tiles measurably more compact there, and cost histogram work), 4 above
+inline int sort_level(long long n) {
+    int l = 1;
+    while (l < kSortMaxLevel && (64LL << (3 * l)) < n)
+        ++l;
+    return (n <= (1LL << 18) && l > 3) ? 3 : l;
+}
+hipError_t launch_sort_points(const SortArgs &a, int n_segs, int n_blks, hipStream_t s);
+
 // adjoint of a direct sum (one pass over the pairs, vjp_kernels.hpp)
 struct VjpArgs {
     const double *sc;  // AoS source coords [n_src*3]
@@ -132,6 +190,9 @@ inline long long default_max_src(long long total_cost, long long concurrent) {
 // Targets per lane of a single-group direct sum: one below 32,768 targets (twice the waves for a
 // small sum), else two.
 inline int direct_targets_per_lane(int n_trg) { return n_trg < 32768 ? 1 : 2; }
+// Direct sums with at least this many sources and targets order their points (sort_points) in the
+// plan-based entries.
+constexpr int kDirectSortMin = 16384;
 
 // lanes per target of a tile of n targets (log2): the largest power of two, at most 16, that still
 // fits the tile's 64*T lanes
@@ -161,6 +222,9 @@ bool uker_has_vjp(int uk);
 constexpr int kPrecMixedFar = 2;
 hipError_t launch_items(int uk, int prec, int T, int grid, const ItemArgs &a, hipStream_t s);
 hipError_t launch_reduce(const ReduceArgs &a, hipStream_t s);
+// the same sums with one thread per target writing all its components (for permuted outputs, whose
+// rows are scattered: one contiguous write per target instead of one per component)
+hipError_t launch_reduce_rows(const ReduceArgs &a, hipStream_t s);
 hipError_t launch_fill_int(int *p, int n, int value, hipStream_t s); // n <= 256
 hipError_t items_occupancy(int uk, int prec, int T, int *blocks_per_cu);
 // scalar-broadcast variant

@@ -30,6 +30,7 @@
 #include "p2p_clear.hpp"
 #include "p2p_internal.hpp"
 #include "p2p_queue.hpp"
+#include "p2p_redo.hpp"
 #include "p2p_sgpr.hpp"
 #include "ukernels.hpp"
 
@@ -113,7 +114,11 @@ __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_ke
     // relative FP32 coordinates of the far-tile form after the values
     constexpr bool FAR = FARB && (PREC != 0) && !HILO && HasUnmaskedEval<K>::value && (STKP2P_FAR_F32 != 0);
     constexpr int FOFF = KOFF + KL;
-    constexpr int KLP = ((KL + KOFF + (FAR ? 3 : 0) + 3) / 4) * 4;
+    // row pitch in floats: whole 16-byte quads, an odd number of them, so that the per-lane row
+    // writes of one staging pass spread over all LDS banks (an even pitch of 4 quads puts the 64
+    // lanes' writes on 4 of the 64 banks); the reads are wave-uniform broadcasts either way
+    constexpr int KLQ = (KL + KOFF + (FAR ? 3 : 0) + 3) / 4;
+    constexpr int KLP = 4 * ((KLQ % 2 == 0) ? KLQ + 1 : KLQ);
     __shared__ double s_x[HILO ? 1 : kSrcTile], s_y[HILO ? 1 : kSrcTile], s_z[HILO ? 1 : kSrcTile];
     __shared__ double s_vd[PREC == 0 ? KL : 1][kSrcTile];
     __shared__ __attribute__((aligned(16))) float s_vf[PREC == 0 ? 1 : kSrcTile][KLP];
@@ -216,6 +221,12 @@ __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_ke
                     }
                     if constexpr (CLEAR)
                         near = box.near(x, y, z);
+                    // mixed: this source's LDS row, assembled in registers and written as 16-byte quads
+                    float row[PREC == 0 ? 1 : KLP];
+                    if constexpr (PREC != 0)
+#pragma unroll
+                        for (int k = 0; k < KLP; ++k)
+                            row[k] = 0.0f;
                     if constexpr (FAR) {
                         const double rx = x - fc[0], ry = y - fc[1], rz = z - fc[2];
                         const double gx = __builtin_fmax(__builtin_fabs(rx) - fh[0], 0.0);
@@ -225,20 +236,20 @@ __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_ke
                         // false for NaN coordinates (near is then true as well)
                         constexpr double R2 = STKP2P_FAR_RATIO * STKP2P_FAR_RATIO;
                         far = !near && lhs * lhs <= R2 * (gx * gx + gy * gy + gz * gz);
-                        s_vf[lane][FOFF] = (float)rx;
-                        s_vf[lane][FOFF + 1] = (float)ry;
-                        s_vf[lane][FOFF + 2] = (float)rz;
+                        row[FOFF] = (float)rx;
+                        row[FOFF + 1] = (float)ry;
+                        row[FOFF + 2] = (float)rz;
                     }
                     double vst[KL];
                     stage_for<K, PREC>(vin, vst);
                     if constexpr (HILO) {
                         const float hx = (float)x, hy = (float)y, hz = (float)z;
-                        s_vf[lane][0] = hx;
-                        s_vf[lane][1] = hy;
-                        s_vf[lane][2] = hz;
-                        s_vf[lane][3] = (float)(x - (double)hx);
-                        s_vf[lane][4] = (float)(y - (double)hy);
-                        s_vf[lane][5] = (float)(z - (double)hz);
+                        row[0] = hx;
+                        row[1] = hy;
+                        row[2] = hz;
+                        row[3] = (float)(x - (double)hx);
+                        row[4] = (float)(y - (double)hy);
+                        row[5] = (float)(z - (double)hz);
                     } else {
                         s_x[lane] = x;
                         s_y[lane] = y;
@@ -246,10 +257,16 @@ __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_ke
                     }
 #pragma unroll
                     for (int k = 0; k < KL; ++k) {
-                        if (PREC == 0)
+                        if constexpr (PREC == 0)
                             s_vd[k][lane] = vst[k];
                         else
-                            s_vf[lane][KOFF + k] = (float)vst[k];
+                            row[KOFF + k] = (float)vst[k];
+                    }
+                    if constexpr (PREC != 0) {
+                        float4 *dst = reinterpret_cast<float4 *>(&s_vf[lane][0]);
+#pragma unroll
+                        for (int q = 0; q < KLP / 4; ++q)
+                            dst[q] = make_float4(row[4 * q], row[4 * q + 1], row[4 * q + 2], row[4 * q + 3]);
                     }
                 }
                 __syncthreads();
@@ -355,6 +372,8 @@ __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_ke
             }
         }
 
+        if constexpr (PREC != 0)
+            redo_nonfinite_fp64<K>(a, item, acc, ls ? sub : 0, split, tx, ty, tz, lam);
         // lanes that share a target add their partial sums (butterfly over the low ls lane bits)
         for (int off = 1; off < split; off <<= 1) {
 #pragma unroll

@@ -37,6 +37,27 @@ __global__ void __launch_bounds__(256) p2p_reduce_kernel(ReduceArgs a) {
     a.out.p[o * a.out.s + k * a.out.fs] += a.scale * s;
 }
 
+// out(perm[i], k) += scale * (slot sums) for every k of target i in one thread: the same sums in the
+// same order as p2p_reduce_kernel (bitwise the same result), one workgroup per tile, tiles dealt to
+// the XCDs in contiguous runs (the scattered rows of neighbouring tiles then share one L2)
+__global__ void __launch_bounds__(256) p2p_reduce_rows_kernel(ReduceArgs a) {
+    const int ti = xcd_unit(blockIdx.x, a.n_tiles);
+    if (ti < 0)
+        return;
+    const Tile tl = a.tiles[ti];
+    const int i = threadIdx.x;
+    if (i >= tl.trg_count)
+        return;
+    const long long row = tl.trg_begin + i;
+    const long long o = a.perm ? (long long)a.perm[row] : row;
+    for (int k = 0; k < a.kt; ++k) {
+        double s = 0.0;
+        for (int slot = tl.slot_begin; slot < tl.slot_end; ++slot)
+            s += a.partial[((long long)slot * a.kt + k) * a.tt + i];
+        a.out.p[o * a.out.s + k * a.out.fs] += a.scale * s;
+    }
+}
+
 __global__ void fill_int_kernel(int *p, int n, int value) {
     if ((int)threadIdx.x < n)
         p[threadIdx.x] = value;
@@ -126,6 +147,13 @@ hipError_t launch_reduce(const ReduceArgs &a, hipStream_t s) {
     if (a.n_tiles <= 0)
         return hipSuccess;
     hipLaunchKernelGGL(p2p_reduce_kernel, dim3(a.n_tiles, a.kt), dim3(a.tt), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_rows(const ReduceArgs &a, hipStream_t s) {
+    if (a.n_tiles <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(p2p_reduce_rows_kernel, dim3((unsigned)xcd_grid(a.n_tiles)), dim3(a.tt), 0, s, a);
     return hipGetLastError();
 }
 

@@ -24,6 +24,7 @@
 #include "p2p_clear.hpp"
 #include "p2p_internal.hpp"
 #include "p2p_queue.hpp"
+#include "p2p_redo.hpp"
 #include "ukernels.hpp"
 
 namespace stkp2p {
@@ -230,6 +231,8 @@ __global__ void __launch_bounds__(kWave) p2p_items_sgpr_kernel(ItemArgs a, const
             }
         }
 
+        if constexpr (PREC != 0)
+            redo_nonfinite_fp64<K>(a, item, acc, sub, split, tx, ty, tz, lam);
         for (int off = 1; off < split; off <<= 1) {
 #pragma unroll
             for (int t = 0; t < T; ++t)
