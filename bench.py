@@ -82,6 +82,19 @@ def roofline(kind, prec, pairs, kernel_ms, kernel_name):
             "flops_per_pair": f, "pairs_per_launch": pairs, "note": NOTE}
 
 
+def attach_traffic(rl, workload, variant):
+    """roofline.traffic from the committed counter summary of this workload's dominant kernel
+    (profiles/<workload>_<variant>_traffic.json: FETCH_SIZE and WRITE_SIZE scaled by the copy
+    calibration), or None when there is none."""
+    tfile = os.path.join(ROOT, "profiles", f"{workload}_{variant}_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            tr = json.load(f)
+        rl["traffic"] = tr.get("hbm_bytes_per_launch")
+        rl["traffic_source"] = os.path.relpath(tfile, ROOT)
+    return rl
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -283,12 +296,7 @@ def ulist_bench(args, torch, S, kind, uker, points, cap, values_fn, kt, metric, 
     far = ",far-tile form" if head == "mixed" and T == 2 else ""
     rl = roofline(kind, "mixed" if head == "mixed" else "fp64", pairs, hv["item_ms"],
                   f"p2p_items_kernel<{int(uker)},{prec_tag},T={T}{far}>")
-    tfile = os.path.join(ROOT, "profiles", f"{workload_name}_{head}_traffic.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            tr = json.load(f)
-        rl["traffic"] = tr.get("hbm_bytes_per_launch")
-        rl["traffic_source"] = os.path.relpath(tfile, ROOT)
+    attach_traffic(rl, workload_name, head)
 
     accuracy, cpu, ref_cpu = None, None, None
     if not args.no_cpu_baseline:
@@ -384,11 +392,14 @@ def lapgrad_direct(args, torch, S, W):
     pairs = len(src) * len(trg)
     # the host-pointer entry (STKFMM::evaluateKernel drop-in): PCIe-inclusive
     h_out = np.zeros((len(trg), 4))
-    S.evaluate_kernel(S.KERNEL.LapPGrad, 0, S.PPKERNEL.SLS2T, src, q, trg, h_out)
-    t = time.perf_counter()
-    for _ in range(5):
+    for _ in range(3):
         S.evaluate_kernel(S.KERNEL.LapPGrad, 0, S.PPKERNEL.SLS2T, src, q, trg, h_out)
-    host_s = (time.perf_counter() - t) / 5
+    calls = []
+    for _ in range(20):
+        t = time.perf_counter()
+        S.evaluate_kernel(S.KERNEL.LapPGrad, 0, S.PPKERNEL.SLS2T, src, q, trg, h_out)
+        calls.append(time.perf_counter() - t)
+    host_s = float(np.median(calls))
     cpu = acc = None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
@@ -414,8 +425,9 @@ def lapgrad_direct(args, torch, S, W):
          "Gpair/s", args, 1e3 * wall / steps, "f64",
          {"workload": "lapgrad-direct-8192x8192", "pairs": pairs, "precision": "fp64"},
          host_api_pcie_inclusive_gpair_s=pairs / host_s / 1e9, tune=tune,
-         roofline=roofline("lapgrad", "fp64", pairs, item_ms,
-                           f"p2p_items_kernel<{int(uker)},FP64,T={tune.get('targets_per_lane', 2)}>"),
+         roofline=attach_traffic(roofline("lapgrad", "fp64", pairs, item_ms,
+                                          f"p2p_items_kernel<{int(uker)},FP64,T={tune.get('targets_per_lane', 2)}>"),
+                                 "lapgrad-direct-8192x8192", "fp64"),
          cpu_baseline=cpu, accuracy_vs_oracle=acc, kernel_ms={"item": item_ms, "reduce": reduce_ms})
 
 
@@ -723,7 +735,7 @@ def stokeslet_instances(args, torch, S, W):
     prec_name = "fp64" if head.startswith("fp64") else "mixed"
     kname = {"mixed-sorted": "p2p_items_kernel<6,MIXED,T=2,far-tile form>", "mixed": "p2p_items_kernel<6,MIXED,T=2>",
              "fp64-sorted": "p2p_items_kernel<6,FP64,T=2>", "fp64": "p2p_items_sgpr_kernel<6,FP64,T=2>"}[head]
-    rl = roofline("stokeslet", prec_name, pairs, h["item_ms"], kname)
+    rl = attach_traffic(roofline("stokeslet", prec_name, pairs, h["item_ms"], kname), "stokeslet-512-instances", head)
     rl["note"] = NOTE + "; kernel_ms = item kernel of an eager execute of the same plan (the replay adds the ordering)"
     cpu = acc = None
     if not args.no_cpu_baseline:
