@@ -6,7 +6,7 @@ import stkfmm_amd as S
 from oracle import oracle as O
 from stkfmm_amd import workloads as W
 from tests.conftest import source_values
-from tests.gpu_util import host, say, to_dev, zeros
+from tests.gpu_util import MIXED_TOL, host, say, to_dev, zeros
 
 pytestmark = pytest.mark.gpu
 
@@ -157,3 +157,28 @@ def test_workload4_adjoint_full_size():
     assert O.max_error(g_val[si], gv_ref)[0] < 1e-11
     assert O.max_error(g_src[si], gs_ref)[0] < 1e-11
     assert O.max_error(g_trg[ti], gt_ref)[0] < 1e-11
+
+
+def test_workload4_adjoint_full_size_mixed():
+    """65,536 x 65,536 PVel, mixed precision: every gradient against the oracle's FP64 gradient on
+    sampled sources and targets (the whole source set against sampled targets and vice versa), to the
+    mixed mode's bar, and the sampled entries of the mixed gradient against the FP64 one."""
+    say("workload 4 full size, mixed")
+    src, val, trg, w = W.adjoint_pvel()
+    ds, dv, dt, dw = to_dev(src), to_dev(val), to_dev(trg), to_dev(w)
+    g_val, g_src, g_trg = (host(x) for x in S.vjp_device(S.UKER.STOKES_PVEL, ds, dv, dt, dw,
+                                                          precision=S.PREC.MIXED))
+    f_val, f_src, f_trg = (host(x) for x in S.vjp_device(S.UKER.STOKES_PVEL, ds, dv, dt, dw))
+    rng = np.random.default_rng(44)
+    si = rng.choice(len(src), 64, replace=False)
+    ti = rng.choice(len(trg), 64, replace=False)
+    gv_ref, gs_ref, _ = O.direct_vjp(7, src[si], val[si], trg, w)
+    _, _, gt_ref = O.direct_vjp(7, src, val, trg[ti], w[ti])
+    for got, ref, what in ((g_val[si], gv_ref, "src_value"), (g_src[si], gs_ref, "src_coord"),
+                           (g_trg[ti], gt_ref, "trg_coord")):
+        l2, mx = O.max_error(got, ref)
+        assert l2 < MIXED_TOL and mx < MIXED_TOL, (what, l2, mx)
+    for got, ref in ((g_val, f_val), (g_src, f_src), (g_trg, f_trg)):
+        assert np.all(np.isfinite(got))
+        l2, mx = O.max_error(got, ref)
+        assert l2 < MIXED_TOL and mx < MIXED_TOL, (l2, mx)

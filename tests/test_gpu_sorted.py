@@ -159,3 +159,38 @@ def test_direct_entry_orders_large_sums(K, p, uk, prec):
     l2, mx = O.max_error(got[rows], O.direct(uk, src, v, trg[rows]))
     tol = FP64_TOL if prec == 0 else MIXED_TOL
     assert l2 < tol and mx < tol, (l2, mx)
+
+
+@pytest.mark.slow
+def test_workload5_ordered_graph_replay_full_size():
+    """All 512 instances (4.7M points), ordered plans in both precisions: graph replays (ordering
+    included) bitwise equal to an eager execute, around unrelated eager work, and the result against
+    the oracle on the largest, the smallest and two other instances."""
+    import torch
+    say("workload 5 full size, ordered, graph replay")
+    sizes, offsets, coord, f = W.stokeslet_instances()
+    d_c, d_f = to_dev(coord), to_dev(f)
+    check = (0, 7, int(np.argmax(sizes)), int(np.argmin(sizes)))
+    for prec in (1, 0):
+        plan = instance_plan(sizes, offsets, prec, tune=SORT)
+        out = zeros(len(coord), 3)
+        plan.execute(d_c, d_f, d_c, out)
+        eager = host(out)
+        tol = FP64_TOL if prec == 0 else MIXED_TOL
+        for i in check:
+            a, b = offsets[i], offsets[i + 1]
+            l2, mx = O.max_error(eager[a:b], O.direct(6, coord[a:b], f[a:b], coord[a:b]))
+            assert l2 < tol and mx < tol, (prec, i, l2, mx)
+        stream = torch.cuda.Stream()
+        out_g = zeros(len(coord), 3)
+        torch.cuda.synchronize()
+        plan.capture_graph(d_c, d_f, d_c, out_g, stream=stream)
+        for rep in range(3):
+            out_g.zero_()
+            torch.cuda.synchronize()
+            plan.launch_graph(stream=stream)
+            stream.synchronize()
+            assert np.array_equal(host(out_g), eager), (prec, rep)
+            tmp = d_f * 3.0
+            del tmp
+        del plan
