@@ -66,8 +66,9 @@ namespace stkp2p {
 // to the tiles that fail the test, which keep the FP64 displacement.  The far-tile loop raises the
 // kernel's register count, which costs occupancy where no tile is far (direct sums of unsorted
 // points: 512 Stokeslet instances measured 42.5 -> 44.8 ms), so it is a separate instantiation
-// (FARB = true, mixed, T = 2) that plans use only for tree-derived work lists (groups with octree
-// levels, whose target and source ranges are spatially compact).
+// (FARB = true, mixed, T = 2) that plans use only where target and source ranges are spatially
+// compact: tree-derived work lists (groups with octree levels) and plans that order their points
+// (morton.hip).
 #ifndef STKP2P_FAR_F32
 #define STKP2P_FAR_F32 1
 #endif
