@@ -913,8 +913,9 @@ int direct_plan(int uk, int prec, int n_src, int n_trg, stkfmm_plan **out) {
     const int gb = 0, ge = n_trg, gp[2] = {0, 1}, rb = 0, re = n_src;
     // one target group.  From 16,384 sources and targets on, the points are ordered along a Morton
     // curve at every call (sort_points, T = 2): measured on MI355X, whole call including the ordering,
-    // 65,536^2 PVel 3.94 -> 3.77 ms FP64 and 2.88 -> 2.70 ms mixed, 131,072^2 Stokeslet mixed 9.97 ->
-    // 8.43 ms, 16,384^2 about even, 8192^2 slower (0.11 -> 0.14 ms: the ordering's fixed cost).
+    // 65,536^2 PVel 3.94 -> 3.77 ms FP64 (LDS staging; 3.76 with scalar broadcast on a box where the
+    // unordered call took 4.03) and 2.88 -> 2.70 ms mixed, 131,072^2 Stokeslet mixed 9.97 -> 8.43 ms,
+    // 16,384^2 about even, 8192^2 slower (0.11 -> 0.14 ms: the ordering's fixed cost).
     // Below that: one target per lane under 32,768 targets (twice the waves for a small sum), and in
     // FP64 the scalar-broadcast staging at or above it (8192^2 LapPGrad 0.127 -> 0.100 ms FP64 and
     // 0.115 -> 0.096 ms mixed at T = 1).  Only the target-to-lane mapping and the staging change, not
@@ -924,9 +925,11 @@ int direct_plan(int uk, int prec, int n_src, int n_trg, stkfmm_plan **out) {
     tn.targets_per_lane = direct_targets_per_lane(n_trg);
     tn.broadcast = (!small && prec == STKFMM_PREC_FP64) ? 1 : 0;
     if (n_trg >= kDirectSortMin && n_src >= kDirectSortMin) {
+        // ordered points: T = 2, and in FP64 the scalar-broadcast records (65,536^2 PVel FP64 4.05 ->
+        // 3.76 ms, 131,072^2 14.72 -> 14.24 ms against LDS staging of the ordered points)
         tn.sort_points = 1;
         tn.targets_per_lane = 2;
-        tn.broadcast = 0;
+        tn.broadcast = (prec == STKFMM_PREC_FP64) ? 1 : 0;
     }
     stkfmm_plan *p = nullptr;
     int rc = stkfmm_plan_create(&p, uk, prec, n_src, n_trg, 1, &gb, &ge, nullptr, gp, &rb, &re, &tn);

@@ -115,11 +115,13 @@ __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_ke
     // relative FP32 coordinates of the far-tile form after the values
     constexpr bool FAR = FARB && (PREC != 0) && !HILO && HasUnmaskedEval<K>::value && (STKP2P_FAR_F32 != 0);
     constexpr int FOFF = KOFF + KL;
-    // row pitch in floats: whole 16-byte quads, an odd number of them, so that the per-lane row
-    // writes of one staging pass spread over all LDS banks (an even pitch of 4 quads puts the 64
-    // lanes' writes on 4 of the 64 banks); the reads are wave-uniform broadcasts either way
+    // row pitch in floats: whole 16-byte quads.  A pitch of a multiple of 4 quads (16 floats) puts the
+    // 64 lanes' row writes of a staging pass on 4 of the 64 LDS banks (16-way conflicts), so it gets
+    // one more quad; other pitches conflict at most 2-way and stay as they are (padding the 2-quad
+    // rows of the PVel single layer measured 8 % slower on its W/X lists; the reads are wave-uniform
+    // broadcasts either way)
     constexpr int KLQ = (KL + KOFF + (FAR ? 3 : 0) + 3) / 4;
-    constexpr int KLP = 4 * ((KLQ % 2 == 0) ? KLQ + 1 : KLQ);
+    constexpr int KLP = 4 * ((KLQ % 4 == 0) ? KLQ + 1 : KLQ);
     __shared__ double s_x[HILO ? 1 : kSrcTile], s_y[HILO ? 1 : kSrcTile], s_z[HILO ? 1 : kSrcTile];
     __shared__ double s_vd[PREC == 0 ? KL : 1][kSrcTile];
     __shared__ __attribute__((aligned(16))) float s_vf[PREC == 0 ? 1 : kSrcTile][KLP];
