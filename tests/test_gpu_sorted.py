@@ -194,3 +194,36 @@ def test_workload5_ordered_graph_replay_full_size():
             tmp = d_f * 3.0
             del tmp
         del plan
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("uk", [13, 6, 11, 0])
+def test_sorted_nan_coordinates_keep_the_mask(uk, prec):
+    """A NaN target coordinate, then a NaN source coordinate, among ordered points: the ordering still
+    places every point (NaN lands in a boundary cell), the NaN pattern of the result equals the
+    oracle's and every other entry is within the precision's bar of it.  (A NaN source turns every
+    target NaN for kernels with r_i factors, as in the reference.)"""
+    say(f"sorted NaN coordinates uker={uk} prec={prec}")
+    rng = np.random.default_rng(950 + uk)
+    ns, nt = 3000, 2000
+    src = rng.random((ns, 3))
+    trg = np.concatenate([src[:5], rng.random((nt - 5, 3))])
+    v = source_values(uk, ns, rng)
+    trg[11, 1] = np.nan
+    src2 = src.copy()
+    src2[7, 0] = np.nan
+    tol = FP64_TOL if prec == 0 else MIXED_TOL
+    for sc in (src, src2):
+        ref = O.direct(uk, sc, v, trg)
+        plan = S.Plan(uk, prec, ns, nt, [0], [nt], None, [0, 1], [0], [ns], SORT)
+        out = zeros(nt, O.trg_dim(uk))
+        if uk == S.UKER.STOKES_PVEL_FUSED:
+            plan.execute(to_dev(sc), to_dev(v[:, :4]), to_dev(trg), out, src_value2=to_dev(v[:, 4:]))
+        else:
+            plan.execute(to_dev(sc), to_dev(v), to_dev(trg), out)
+        got = host(out)
+        assert np.array_equal(np.isnan(got), np.isnan(ref))
+        rows = ~np.isnan(ref).any(axis=1)
+        if rows.any():
+            l2, mx = O.max_error(got[rows], ref[rows])
+            assert l2 < tol and mx < tol, (l2, mx)
