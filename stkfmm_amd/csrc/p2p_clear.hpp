@@ -5,8 +5,10 @@
 // a range in the scalar-broadcast kernel) is checked one source per lane with one ballot: when every
 // source lies outside the box by ClearGap on some axis, every pair of the group has r2 > 0 and the
 // group runs the micro-kernel's unmasked form (eval<R, false>), which gives the same results without
-// the compare and select of each pair.  NaN coordinates make the box or the source test fail, so such
-// groups keep the mask.
+// the compare and select of each pair.  A NaN coordinate makes r2 NaN, which the mask turns into a
+// zero contribution, so it must never pass: a NaN in any target's coordinates makes the box the whole
+// space, and a source with a NaN coordinate counts as near however far its other coordinates are
+// (a separation on one axis says nothing about a NaN on another).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -71,15 +73,20 @@ struct TargetBox {
                 hi[d] = max_nan(hi[d], __shfl_xor(hi[d], off));
             }
         }
+        // NaN propagates through the reductions: a NaN target makes some bound NaN, and then every
+        // source is near
+        const bool nan_box = lo[0] != lo[0] || lo[1] != lo[1] || lo[2] != lo[2] || hi[0] != hi[0] ||
+                             hi[1] != hi[1] || hi[2] != hi[2];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            lo[d] = uniform_f64(lo[d] - ClearGap<PREC>::value);
-            hi[d] = uniform_f64(hi[d] + ClearGap<PREC>::value);
+            lo[d] = uniform_f64(nan_box ? -__builtin_inf() : lo[d] - ClearGap<PREC>::value);
+            hi[d] = uniform_f64(nan_box ? __builtin_inf() : hi[d] + ClearGap<PREC>::value);
         }
     }
-    // the source may coincide with a target (not separated on any axis, or NaN somewhere)
+    // the source may coincide with a target: not separated on any axis, or a NaN coordinate
     __device__ bool near(double x, double y, double z) const {
-        return !(x < lo[0] || x > hi[0] || y < lo[1] || y > hi[1] || z < lo[2] || z > hi[2]);
+        const bool sep = x < lo[0] || x > hi[0] || y < lo[1] || y > hi[1] || z < lo[2] || z > hi[2];
+        return !sep || x != x || y != y || z != z;
     }
 };
 

@@ -196,14 +196,18 @@ def test_workload5_ordered_graph_replay_full_size():
         del plan
 
 
+@pytest.mark.parametrize("bc", [0, 1], ids=["lds", "scalar"])
 @pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
 @pytest.mark.parametrize("uk", [13, 6, 11, 0])
-def test_sorted_nan_coordinates_keep_the_mask(uk, prec):
+def test_sorted_nan_coordinates_keep_the_mask(uk, prec, bc):
     """A NaN target coordinate, then a NaN source coordinate, among ordered points: the ordering still
     places every point (NaN lands in a boundary cell), the NaN pattern of the result equals the
     oracle's and every other entry is within the precision's bar of it.  (A NaN source turns every
-    target NaN for kernels with r_i factors, as in the reference.)"""
-    say(f"sorted NaN coordinates uker={uk} prec={prec}")
+    target NaN for kernels with r_i factors, as in the reference.)  Ordered target tiles are compact
+    boxes, so a point with one NaN coordinate is often separated from a tile on another axis; the
+    Laplace potential (uker 0) is where the unmasked form would show it, as NaN where the reference's
+    mask gives 0."""
+    say(f"sorted NaN coordinates uker={uk} prec={prec} broadcast={bc}")
     rng = np.random.default_rng(950 + uk)
     ns, nt = 3000, 2000
     src = rng.random((ns, 3))
@@ -215,7 +219,7 @@ def test_sorted_nan_coordinates_keep_the_mask(uk, prec):
     tol = FP64_TOL if prec == 0 else MIXED_TOL
     for sc in (src, src2):
         ref = O.direct(uk, sc, v, trg)
-        plan = S.Plan(uk, prec, ns, nt, [0], [nt], None, [0, 1], [0], [ns], SORT)
+        plan = S.Plan(uk, prec, ns, nt, [0], [nt], None, [0, 1], [0], [ns], dict(SORT, broadcast=bc))
         out = zeros(nt, O.trg_dim(uk))
         if uk == S.UKER.STOKES_PVEL_FUSED:
             plan.execute(to_dev(sc), to_dev(v[:, :4]), to_dev(trg), out, src_value2=to_dev(v[:, 4:]))
