@@ -275,7 +275,9 @@ __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_ke
                 __syncthreads();
                 // sources of this tile: every lane takes all of them (split == 1, uniform loop with
                 // immediate LDS offsets) or every split-th one starting at its sub-lane
-                auto tile_pairs = [&](auto strided, auto clear, auto farc) {
+                // FP64: the whole tile, sources [jb, je) = [0, n); mixed: one block of kMixedAccBlock
+                // sources [jb, je), whose FP32 partial sums then join the FP64 accumulators
+                auto tile_pairs = [&](auto strided, auto clear, auto farc, int jb, int je) {
                     constexpr bool STR = decltype(strided)::value;
                     constexpr bool MASK = !decltype(clear)::value;
                     constexpr bool FARM = decltype(farc)::value;
@@ -283,7 +285,7 @@ __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_ke
                     const int j_step = STR ? split : 1;
                     if (PREC == 0) {
 #pragma unroll 2
-                        for (int j = j_first; j < n; j += j_step) {
+                        for (int j = jb + j_first; j < je; j += j_step) {
                             const double sx = s_x[j], sy = s_y[j], sz = s_z[j];
                             double f[KL];
 #pragma unroll
@@ -294,8 +296,8 @@ __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_ke
                                 eval_pair<K, MASK>(acc[t], tx[t] - sx, ty[t] - sy, tz[t] - sz, f);
                         }
                     } else {
-                        for (int j0 = 0; j0 < n; j0 += kMixedAccBlock) {
-                            const int j1 = (n - j0 < kMixedAccBlock) ? n : j0 + kMixedAccBlock;
+                        {
+                            const int j0 = jb, j1 = je;
                             float part[T][KT];
 #pragma unroll
                             for (int t = 0; t < T; ++t)
@@ -352,25 +354,44 @@ __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_ke
                         }
                     }
                 };
-                // uniform: no lane holds a source near the targets; every source passes the far test
-                const bool clear = CLEAR && !__any(near);
-                const bool far_tile = FAR && clear && __all(far);
                 using F = std::false_type;
                 using Tr = std::true_type;
-                if (ls == 0) {
-                    if (far_tile)
-                        tile_pairs(F{}, Tr{}, std::integral_constant<bool, FAR>{});
-                    else if (clear)
-                        tile_pairs(F{}, Tr{}, F{});
-                    else
-                        tile_pairs(F{}, F{}, F{});
+                // sources [jb, je) in the form their flags allow: uniform decisions, one per call
+                auto run = [&](int jb, int je, bool clear, bool far_tile) {
+                    if (ls == 0) {
+                        if (far_tile)
+                            tile_pairs(F{}, Tr{}, std::integral_constant<bool, FAR>{}, jb, je);
+                        else if (clear)
+                            tile_pairs(F{}, Tr{}, F{}, jb, je);
+                        else
+                            tile_pairs(F{}, F{}, F{}, jb, je);
+                    } else {
+                        if (far_tile)
+                            tile_pairs(Tr{}, Tr{}, std::integral_constant<bool, FAR>{}, jb, je);
+                        else if (clear)
+                            tile_pairs(Tr{}, Tr{}, F{}, jb, je);
+                        else
+                            tile_pairs(Tr{}, F{}, F{}, jb, je);
+                    }
+                };
+                if constexpr (PREC == 0) {
+                    // uniform: no lane holds a source near the targets
+                    run(0, n, CLEAR && !__any(near), false);
                 } else {
-                    if (far_tile)
-                        tile_pairs(Tr{}, Tr{}, std::integral_constant<bool, FAR>{});
-                    else if (clear)
-                        tile_pairs(Tr{}, Tr{}, F{});
-                    else
-                        tile_pairs(Tr{}, F{}, F{});
+                    // per block of kMixedAccBlock sources (lane j holds source j's flags; lanes past n
+                    // hold no source and count as clear and far): a block with no source near the
+                    // targets drops the mask, and one whose every source passes the far test takes
+                    // the FP32 displacement -- a finer choice than the whole tile's, which on the 1M
+                    // PVel U-list moves ~10 % of the pairs from clear to far tiles
+                    const unsigned long long nm = __ballot(near), fm = __ballot(far);
+                    constexpr unsigned long long BM =
+                        (kMixedAccBlock >= 64) ? ~0ull : ((1ull << kMixedAccBlock) - 1ull);
+                    for (int j0 = 0; j0 < n; j0 += kMixedAccBlock) {
+                        const unsigned long long bm = BM << j0;
+                        const bool bclear = CLEAR && (nm & bm) == 0ull;
+                        const bool bfar = FAR && bclear && (fm & bm) == bm;
+                        run(j0, (n - j0 < kMixedAccBlock) ? n : j0 + kMixedAccBlock, bclear, bfar);
+                    }
                 }
             }
         }
