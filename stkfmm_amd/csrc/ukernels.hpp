@@ -285,7 +285,9 @@ struct UkLaplaceDipolePGradGrad {
 // ---- Stokes -----------------------------------------------------------------------------------
 
 // Stokeslet 3 -> 3 (PVFMM stokes_vel, StokesLayerKernel.hpp:38-42; closed form of
-// Test/SimpleKernel.cpp:986-1023 with a = 0): u = f/r + r (f.r)/r^3.
+// Test/SimpleKernel.cpp:986-1023 with a = 0): u = f/r + r (f.r)/r^3, evaluated as
+// u_i = (1/r) (f_i + r_i h) with h = (f.r)/r^2: one multiply per pair fewer than f_i/r + r_i (f.r)/r^3
+// (no 1/r^3), the same two FMAs per component.
 struct UkStokesVel {
     static constexpr int KS = 3, KS1 = 3, KL = 3, KT = 3;
     static constexpr bool UNMASKED_EVAL = true;
@@ -298,15 +300,10 @@ struct UkStokesVel {
         R r2 = dx * dx + dy * dy + dz * dz;
         const auto rs = pair_rsqrt<M>(r2);
         R rinv = rs.s;
-        R rinv3 = rs.s2 * rinv;
-        R fdotr = f[0] * dx + f[1] * dy + f[2] * dz;
-        R c = fdotr * rinv3;
-        u[0] += f[0] * rinv;
-        u[0] += dx * c;
-        u[1] += f[1] * rinv;
-        u[1] += dy * c;
-        u[2] += f[2] * rinv;
-        u[2] += dz * c;
+        R h = (f[0] * dx + f[1] * dy + f[2] * dz) * rs.s2;
+        u[0] += rinv * (f[0] + dx * h);
+        u[1] += rinv * (f[1] + dy * h);
+        u[2] += rinv * (f[2] + dz * h);
     }
 };
 
