@@ -1,8 +1,9 @@
-// ukernel_host.cpp -- test harness: the library's per-pair micro-kernels (stage, eval, post) compiled
-// for the CPU, one pair at a time, so tests/test_ukernels_host.py can check their algebra against the
-// oracle in FP64 and FP32 without a GPU.  Built by stkfmm_amd/csrc/Makefile with the host shim in
-// tests/cxx/shim.  Not part of the library.
+// ukernel_host.cpp -- test harness: the library's per-pair micro-kernels (stage, eval, post) and
+// per-pair adjoints compiled for the CPU, one pair at a time, so tests/test_ukernels_host.py can check
+// their algebra against the oracle in FP64 and FP32 without a GPU.  Built by
+// stkfmm_amd/csrc/Makefile with the host shim in tests/cxx/shim.  Not part of the library.
 #include "../../stkfmm_amd/csrc/ukernels_family.hpp"
+#include "../../stkfmm_amd/csrc/vjp_pairs.hpp"
 
 using namespace stkp2p;
 
@@ -41,7 +42,49 @@ int run(int prec, const double *r, const double *f, double *u) {
     }
     return K::KT;
 }
+
+// one pair through the adjoint V::vjp in R arithmetic; gr[3] and gf[KS] are overwritten
+template <class V, class R>
+int vjp_run(const double *r, const double *f, const double *w, double *gr, double *gf) {
+    R fr[V::KS], wr[V::KT], g[3] = {R(0), R(0), R(0)}, h[V::KS];
+    for (int k = 0; k < V::KS; ++k) {
+        fr[k] = (R)f[k];
+        h[k] = R(0);
+    }
+    for (int k = 0; k < V::KT; ++k)
+        wr[k] = (R)w[k];
+    V::template vjp<R>((R)r[0], (R)r[1], (R)r[2], fr, wr, g, h);
+    for (int d = 0; d < 3; ++d)
+        gr[d] = (double)g[d];
+    for (int k = 0; k < V::KS; ++k)
+        gf[k] = (double)h[k];
+    return V::KS;
+}
+
+template <class V>
+int vjp_prec(int prec, const double *r, const double *f, const double *w, double *gr, double *gf) {
+    return prec == 0 ? vjp_run<V, double>(r, f, w, gr, gf) : vjp_run<V, float>(r, f, w, gr, gf);
+}
 } // namespace
+
+// g_r (3) and g_f (KS values) of one pair (no ScaleFactor) through the library's adjoint of micro-
+// kernel uk, prec 0 = FP64, 1 = FP32 arithmetic; returns KS, or -1 when uk has no adjoint
+extern "C" int vjp_host_pair(int uk, int prec, const double *r, const double *f, const double *w, double *gr,
+                             double *gf) {
+    switch (uk) {
+    case 0: return vjp_prec<VjpLaplaceP>(prec, r, f, w, gr, gf);
+    case 2: return vjp_prec<VjpLaplacePGrad>(prec, r, f, w, gr, gf);
+    case 6: return vjp_prec<VjpStokesVel>(prec, r, f, w, gr, gf);
+    case 7: return vjp_prec<VjpStokesPVel>(prec, r, f, w, gr, gf);
+    case 8: return vjp_prec<VjpStokesDoublePVel>(prec, r, f, w, gr, gf);
+    case 9: return vjp_prec<VjpStokesRegVel>(prec, r, f, w, gr, gf);
+    case 10: return vjp_prec<VjpRpyU>(prec, r, f, w, gr, gf);
+    case 11: return vjp_prec<VjpRpyULapU>(prec, r, f, w, gr, gf);
+    case 12: return vjp_prec<VjpStkULapU>(prec, r, f, w, gr, gf);
+    case 13: return vjp_prec<VjpStokesPVelFused>(prec, r, f, w, gr, gf);
+    default: return -1;
+    }
+}
 
 // u = the micro-kernel's sum for one pair (no ScaleFactor), prec 0 = FP64, 1 = FP32 arithmetic,
 // 2 / 3 = the same without the coincidence mask (eval<R, false>, for pairs with r2 > 0); returns the

@@ -138,3 +138,53 @@ def test_nan_displacement_is_masked(prec):
     # is 0 (kernels with r_i factors give NaN, as the reference does)
     for r in ([np.nan, 0.1, 0.2], [0.3, np.nan, np.nan]):
         assert np.array_equal(host_pair(0, prec, np.array(r), np.array([1.5])), [0.0])
+
+
+VJP_UKERS = (0, 2, 6, 7, 8, 9, 10, 11, 12, 13)
+
+
+def host_vjp(uk, prec, r, f, w):
+    L = lib()
+    L.vjp_host_pair.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+    r, f, w = (np.ascontiguousarray(a, dtype=np.float64) for a in (r, f, w))
+    gr, gf = np.zeros(3), np.zeros(32)
+    ks = L.vjp_host_pair(uk, prec, r.ctypes.data_as(_dp), f.ctypes.data_as(_dp), w.ctypes.data_as(_dp),
+                         gr.ctypes.data_as(_dp), gf.ctypes.data_as(_dp))
+    assert ks == O.src_dim(uk)
+    return gr, gf[:ks]
+
+
+@pytest.mark.parametrize("uk", VJP_UKERS)
+def test_vjp_pairs_match_oracle(uk):
+    # the per-pair adjoints of the device kernels (stkfmm_amd/csrc/vjp_pairs.hpp) against the oracle's
+    # (which is pinned against central finite differences of the forward): FP64 to 1e-12 of the
+    # gradient's largest entry, FP32 to FP32 rounding amplified by the cancellation of the position
+    # gradient's terms; pairs from 1e-3 to 10 apart
+    rng = np.random.default_rng(1200 + uk)
+    for _ in range(200):
+        r = rng.uniform(-1, 1, 3) * 10 ** rng.uniform(-3, 1)
+        f = values(uk, rng)
+        w = rng.uniform(-1, 1, O.trg_dim(uk))
+        gr_o, gf_o = O.vjp_pair(uk, r, f, w)
+        ref = np.concatenate([gr_o, gf_o])
+        scale = np.abs(ref).max()
+        for prec, tol in ((0, 1e-12), (1, 2e-5)):
+            gr, gf = host_vjp(uk, prec, r, f, w)
+            err = np.abs(np.concatenate([gr, gf]) - ref).max() / scale
+            assert err <= tol, (prec, r, err)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_vjp_coincident_pair(prec):
+    # the forward is masked at r = 0, so the pair's adjoint is exactly zero -- except the regularised
+    # Stokeslet (uker 9), finite there (rho^2 = r^2 + eps^2 > 0): its adjoint matches the oracle's
+    rng = np.random.default_rng(1300)
+    for uk in VJP_UKERS:
+        f, w = values(uk, rng), rng.uniform(-1, 1, O.trg_dim(uk))
+        gr, gf = host_vjp(uk, prec, np.zeros(3), f, w)
+        if uk == 9:
+            ref = np.concatenate(O.vjp_pair(uk, np.zeros(3), f, w))
+            tol = 1e-12 if prec == 0 else 2e-5
+            assert np.abs(np.concatenate([gr, gf]) - ref).max() <= tol * np.abs(ref).max()
+        else:
+            assert not gr.any() and not gf.any(), uk
