@@ -511,7 +511,7 @@ def pvel_adjoint(args, torch, S, W):
                        note=NOTE + "; kernel_ms = the adjoint's kernels (pair pass + partial-sum reductions) "
                                    "between HIP events on their stream"),
          roofline_mixed=roofline("pvel_vjp", "mixed", pairs, adj_mixed_kernel_ms,
-                                 "vjp_pair_kernel<stokes_pvel,MIXED,T=2>"),
+                                 "vjp_pair_kernel<stokes_pvel,MIXED,T=4>"),
          cpu_baseline=cpu, accuracy_vs_oracle=acc)
 
 

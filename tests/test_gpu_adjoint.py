@@ -51,6 +51,40 @@ def test_vjp_multi_tile_waves(uk, prec, monkeypatch):
         assert l2 < tol and mx < tol, (name, l2, mx)
 
 
+@pytest.mark.parametrize("uk", [0, 2, 6, 7, 8, 9, 10, 11, 12, 13])
+def test_vjp_mixed_four_targets_per_lane(uk):
+    # mixed mode takes 4 targets per lane from 4,096 targets: 4,133 targets (a ragged last tile of 37)
+    # against 700 sources (a ragged last source tile), every micro-kernel with an adjoint
+    say(f"vjp mixed T=4 uker={uk} ns=700 nt=4133")
+    rng = np.random.default_rng(750 + uk)
+    ns, nt = 700, 4133
+    src = rng.random((ns, 3))
+    trg = np.concatenate([src[:11], rng.random((nt - 11, 3))])
+    v = source_values(uk, ns, rng)
+    w = rng.normal(size=(nt, O.trg_dim(uk)))
+    ref = O.direct_vjp(uk, src, v, trg, w)
+    got = S.vjp_device(uk, to_dev(src), to_dev(v), to_dev(trg), to_dev(w), precision=S.PREC.MIXED)
+    for g, r, name in zip(got, ref, ("g_src_value", "g_src_coord", "g_trg_coord")):
+        l2, mx = O.max_error(host(g), r)
+        assert l2 < 1e-5 and mx < 1e-5, (name, l2, mx)
+
+
+def test_vjp_targets_per_lane_override_is_checked(monkeypatch):
+    # STKFMM_VJP_T: 1, 2, or 4 in mixed mode; anything else is refused with an argument error
+    rng = np.random.default_rng(760)
+    src, trg = to_dev(rng.random((70, 3))), to_dev(rng.random((80, 3)))
+    v, w = to_dev(rng.normal(size=(70, 4))), to_dev(rng.normal(size=(80, 4)))
+    for bad, prec in (("3", S.PREC.MIXED), ("4", S.PREC.FP64), ("0", S.PREC.MIXED)):
+        monkeypatch.setenv("STKFMM_VJP_T", bad)
+        with pytest.raises(S.ArgumentError):
+            S.vjp_device(S.UKER.STOKES_PVEL, src, v, trg, w, precision=prec)
+    monkeypatch.setenv("STKFMM_VJP_T", "4")
+    got = S.vjp_device(S.UKER.STOKES_PVEL, src, v, trg, w, precision=S.PREC.MIXED)
+    ref = O.direct_vjp(7, host(src), host(v), host(trg), host(w))
+    for g, r in zip(got, ref):
+        assert O.max_error(host(g), r)[1] < 1e-5
+
+
 def test_autograd_function_matches_vjp():
     import torch
     from stkfmm_amd.autograd import p2p
