@@ -382,7 +382,7 @@ def lapgrad_direct(args, torch, S, W):
     # a single 8192 x 8192 sum is small for 256 CUs: tune the item size down as well
     from stkfmm_amd.autotune import autotune
     cands = [dict(targets_per_lane=t, max_src_per_item=m, persistent=p, broadcast=b)
-             for t in (1, 2, 4) for m in (256, 512, 1024, 2048) for p in (1, 2) for b in (0, 1)]
+             for t in (1, 2, 4) for m in (256, 320, 384, 448, 512, 1024, 2048) for p in (1, 2) for b in (0, 1)]
     scratch = torch.zeros_like(out)
     tune, _ = autotune(work, uker, S.PREC.FP64, dict(kw, trg_value=scratch, stream=stream), candidates=cands)
     del scratch

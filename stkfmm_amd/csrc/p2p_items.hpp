@@ -69,6 +69,14 @@ namespace stkp2p {
 // (FARB = true, mixed, T = 2) that plans use only where target and source ranges are spatially
 // compact: tree-derived work lists (groups with octree levels) and plans that order their points
 // (morton.hip).
+// STKP2P_FP64_UNROLL_T1 (build option, default 4): unroll of the FP64 source loop at one target per
+// lane (small direct sums, whose few waves per SIMD leave the FP64 dependency chains exposed; more
+// sources in flight per wave hide them).  8192^2 LapPGrad item kernel, same box, twice each: unroll 2
+// 71.5 / 71.2 us, 4 67.6 / 68.4 us, 8 69.7 / 69.6 us.  The sums are added in the same order whatever
+// the unroll.
+#ifndef STKP2P_FP64_UNROLL_T1
+#define STKP2P_FP64_UNROLL_T1 4
+#endif
 #ifndef STKP2P_FAR_F32
 #define STKP2P_FAR_F32 1
 #endif
@@ -284,7 +292,8 @@ __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_ke
                     const int j_first = STR ? sub : 0;
                     const int j_step = STR ? split : 1;
                     if (PREC == 0) {
-#pragma unroll 2
+                        constexpr int U64 = (T == 1) ? STKP2P_FP64_UNROLL_T1 : 2;
+#pragma unroll U64
                         for (int j = jb + j_first; j < je; j += j_step) {
                             const double sx = s_x[j], sy = s_y[j], sz = s_z[j];
                             double f[KL];
