@@ -153,6 +153,19 @@ def test_workload2_pvel_ulist_full_size():
     # whole-run mixed vs FP64 agreement
     l2, mx = O.max_error(outs[1], outs[0])
     assert l2 < MIXED_TOL, (l2, mx)
+    # the shipped tunings the bench runs (stkfmm_amd/tuned_defaults.json: FP64 scalar-broadcast
+    # records, T = 2, 2048 sources per item; mixed LDS staging, far-tile form): the same leaves
+    # against the oracle, and the whole run against the default plan's
+    for prec, tol in ((0, FP64_TOL), (1, MIXED_TOL)):
+        say(f"workload 2 prec={prec} shipped tuning")
+        plan = S.Plan.from_tree(tr, 13, prec, tune="auto")
+        out = zeros(tr.n, 4)
+        plan.execute(d_x, d_sl, d_x, out, src_value2=d_dl)
+        got = host(out)
+        l2, mx = O.max_error(got[rows], ref[rows])
+        assert l2 < tol and mx < tol, (prec, l2, mx)
+        l2, mx = O.max_error(got, outs[prec])
+        assert l2 < tol, (prec, l2, mx)
     # linearity: doubling every strength doubles the FP64 result exactly
     plan = S.Plan.from_tree(tr, 13, 0)
     out2 = zeros(tr.n, 4)
@@ -171,8 +184,8 @@ def test_workload3_rpy_full_size():
     d_x, d_v = to_dev(xs), to_dev(v)
     outs = {}
     for prec in (1, 0):
-        say(f"workload 3 prec={prec}")
-        plan = S.Plan.from_tree(tr, 11, prec)
+        say(f"workload 3 prec={prec} shipped tuning")
+        plan = S.Plan.from_tree(tr, 11, prec, tune="auto")  # the tunings the bench runs
         out = zeros(tr.n, 6)
         plan.execute(d_x, d_v, d_x, out)
         outs[prec] = host(out)
@@ -198,8 +211,8 @@ def test_workload2_pvel_ulist_4m():
     d_x, d_sl, d_dl = to_dev(xs), to_dev(sl), to_dev(dl)
     outs = {}
     for prec in (1, 0):
-        say(f"workload 2 at 4M prec={prec}")
-        plan = S.Plan.from_tree(tr, 13, prec)
+        say(f"workload 2 at 4M prec={prec} shipped tuning")
+        plan = S.Plan.from_tree(tr, 13, prec, tune="auto")  # the tunings the bench runs
         out = zeros(tr.n, 4)
         plan.execute(d_x, d_sl, d_x, out, src_value2=d_dl)
         outs[prec] = host(out)
