@@ -153,6 +153,16 @@ def test_workload2_pvel_ulist_full_size():
     # whole-run mixed vs FP64 agreement
     l2, mx = O.max_error(outs[1], outs[0])
     assert l2 < MIXED_TOL, (l2, mx)
+    # every target of the 1M run against the oracle's vectorised FP64 U-list (oracle_ranges_simd,
+    # pinned against the scalar oracle_ranges by tests/test_oracle_drivers.py; ~1e10 pairs, a few
+    # seconds on the host's threads)
+    say("workload 2 whole-run oracle")
+    ref_all = O.ranges(13, xs, np.concatenate([sl, dl], axis=1), xs, tr.leaf_begin, tr.leaf_end, tr.ulist_ptr,
+                       tr.rng_begin, tr.rng_end, simd=True)
+    l2, mx = O.max_error(outs[0], ref_all)
+    assert l2 < FP64_TOL and mx < FP64_TOL, (l2, mx)
+    l2, mx = O.max_error(outs[1], ref_all)
+    assert l2 < MIXED_TOL and mx < MIXED_TOL, (l2, mx)
     # the shipped tunings the bench runs (stkfmm_amd/tuned_defaults.json: FP64 scalar-broadcast
     # records, T = 2, 2048 sources per item; mixed LDS staging, far-tile form): the same leaves
     # against the oracle, and the whole run against the default plan's
