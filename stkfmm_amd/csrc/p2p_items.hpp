@@ -104,6 +104,15 @@ template <class K, class = void> struct MixedMinWaves { static constexpr int val
 template <class K> struct MixedMinWaves<K, decltype((void)K::MIXED_MIN_WAVES)> {
     static constexpr int value = K::MIXED_MIN_WAVES;
 };
+// sources per FP32 partial sum in mixed mode (a multiple of kMixedAccBlock that divides the 64-source
+// tile): MIXED_ACC_SOURCES of the micro-kernel, else kMixedAccBlock.  The path (masked / clear / far)
+// is still chosen per kMixedAccBlock sources.
+template <class K, class = void> struct MixedFlush { static constexpr int value = kMixedAccBlock; };
+template <class K> struct MixedFlush<K, decltype((void)K::MIXED_ACC_SOURCES)> {
+    static constexpr int value = K::MIXED_ACC_SOURCES;
+    static_assert(value % kMixedAccBlock == 0 && kSrcTile % value == 0, "MIXED_ACC_SOURCES");
+};
+
 template <int UK, int PREC> struct ItemMinWaves {
     static constexpr int value = (PREC != 0) ? MixedMinWaves<typename UkerOf<UK>::type>::value : 1;
 };
@@ -285,6 +294,14 @@ __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_ke
                 // immediate LDS offsets) or every split-th one starting at its sub-lane
                 // FP64: the whole tile, sources [jb, je) = [0, n); mixed: one block of kMixedAccBlock
                 // sources [jb, je), whose FP32 partial sums then join the FP64 accumulators
+                // mixed: FP32 partial sums of the current group of MixedFlush<K> sources, added into the
+                // FP64 accumulators when the group ends
+                float part[T][KT];
+#pragma unroll
+                for (int t = 0; t < T; ++t)
+#pragma unroll
+                    for (int k = 0; k < KT; ++k)
+                        part[t][k] = 0.0f;
                 auto tile_pairs = [&](auto strided, auto clear, auto farc, int jb, int je) {
                     constexpr bool STR = decltype(strided)::value;
                     constexpr bool MASK = !decltype(clear)::value;
@@ -307,12 +324,6 @@ __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_ke
                     } else {
                         {
                             const int j0 = jb, j1 = je;
-                            float part[T][KT];
-#pragma unroll
-                            for (int t = 0; t < T; ++t)
-#pragma unroll
-                                for (int k = 0; k < KT; ++k)
-                                    part[t][k] = 0.0f;
                             // sources per iteration: MixedUnroll<K> (measured on MI355X, T = 2: the
                             // fused PVel U-list 8.37 ms at 1, 8.62 at 2, 8.91 at 4; RPY 1 % faster
                             // at 2 than at 1)
@@ -355,11 +366,6 @@ __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_ke
                                                            (float)(tz[t] - sz), f);
                                 }
                             }
-#pragma unroll
-                            for (int t = 0; t < T; ++t)
-#pragma unroll
-                                for (int k = 0; k < KT; ++k)
-                                    acc[t][k] += (double)part[t][k];
                         }
                     }
                 };
@@ -395,11 +401,22 @@ __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_ke
                     const unsigned long long nm = __ballot(near), fm = __ballot(far);
                     constexpr unsigned long long BM =
                         (kMixedAccBlock >= 64) ? ~0ull : ((1ull << kMixedAccBlock) - 1ull);
+                    constexpr int FL = MixedFlush<K>::value;
                     for (int j0 = 0; j0 < n; j0 += kMixedAccBlock) {
                         const unsigned long long bm = BM << j0;
                         const bool bclear = CLEAR && (nm & bm) == 0ull;
                         const bool bfar = FAR && bclear && (fm & bm) == bm;
-                        run(j0, (n - j0 < kMixedAccBlock) ? n : j0 + kMixedAccBlock, bclear, bfar);
+                        const int j1 = (n - j0 < kMixedAccBlock) ? n : j0 + kMixedAccBlock;
+                        run(j0, j1, bclear, bfar);
+                        if (j1 % FL == 0 || j1 == n) {
+#pragma unroll
+                            for (int t = 0; t < T; ++t)
+#pragma unroll
+                                for (int k = 0; k < KT; ++k) {
+                                    acc[t][k] += (double)part[t][k];
+                                    part[t][k] = 0.0f;
+                                }
+                        }
                     }
                 }
             }

@@ -534,6 +534,10 @@ struct UkRpyULapU {
     static constexpr int KS = 4, KS1 = 4, KL = 8, KT = 6;
     static constexpr bool UNMASKED_EVAL = true;
     static constexpr int MIXED_MIN_WAVES = 4; // item kernel's mixed-mode occupancy request (measured)
+    // FP32 partial sums over 32 sources (the form is still chosen per 16): the FP64 flush of the six
+    // outputs costs less; 4M RPY U-list, same box, twice each: 40.68 -> 40.29 ms, errorL2 4.24e-7 ->
+    // 3.16e-7, maxrel 3.38e-7 -> 3.10e-7
+    static constexpr int MIXED_ACC_SOURCES = 32;
     static constexpr unsigned SRC_SCALE_MASK = 0x8u;
     __host__ __device__ static constexpr int deg(int k) { return k < 3 ? 1 : 3; }
     __host__ __device__ static constexpr double post(int k) { return k < 3 ? 1.0 : -6.0; }
