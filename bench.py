@@ -507,11 +507,13 @@ def pvel_adjoint(args, torch, S, W):
          forward_ms=1e3 * fwd, adjoint_ms=1e3 * adj, autograd_fwd_bwd_ms=1e3 * ag,
          forward_gpair_s=pairs / fwd / 1e9, adjoint_gpair_s=pairs / adj / 1e9,
          adjoint_mixed_ms=1e3 * adj_mixed, adjoint_mixed_gpair_s=pairs / adj_mixed / 1e9,
-         roofline=dict(roofline("pvel_vjp", "fp64", pairs, adj_kernel_ms, "vjp_pair_kernel<stokes_pvel,FP64,T=2>"),
-                       note=NOTE + "; kernel_ms = the adjoint's kernels (pair pass + partial-sum reductions) "
-                                   "between HIP events on their stream"),
-         roofline_mixed=roofline("pvel_vjp", "mixed", pairs, adj_mixed_kernel_ms,
-                                 "vjp_pair_kernel<stokes_pvel,MIXED,T=4>"),
+         roofline=attach_traffic(
+             dict(roofline("pvel_vjp", "fp64", pairs, adj_kernel_ms, "vjp_pair_kernel<stokes_pvel,FP64,T=2>"),
+                  note=NOTE + "; kernel_ms = the adjoint's kernels (pair pass + partial-sum reductions) "
+                              "between HIP events on their stream"), "pvel-adjoint-65536x65536", "fp64"),
+         roofline_mixed=attach_traffic(roofline("pvel_vjp", "mixed", pairs, adj_mixed_kernel_ms,
+                                                "vjp_pair_kernel<stokes_pvel,MIXED,T=4>"),
+                                       "pvel-adjoint-65536x65536", "mixed"),
          cpu_baseline=cpu, accuracy_vs_oracle=acc)
 
 
@@ -576,8 +578,9 @@ def pvel_fmm_lists(args, torch, S, W):
     ms_mixed = sum(v["mixed"]["ms_per_step"] for v in per.values())
     ms_fp64 = sum(v["fp64"]["ms_per_step"] for v in per.values())
     big = max(per, key=lambda k: per[k]["pairs"])
-    rl = roofline("pvel_sl", "mixed", per[big]["pairs"], per[big]["mixed"]["item_ms"],
-                  f"p2p_items_kernel<{int(uk)},MIXED,T=2> ({big} list)")
+    rl = attach_traffic(roofline("pvel_sl", "mixed", per[big]["pairs"], per[big]["mixed"]["item_ms"],
+                                 f"p2p_items_kernel<{int(uk)},MIXED,T=2> ({big} list)"),
+                        "pvel-fmm-lists-1M-clustered-leafcap1000-p8", f"{big}-mixed")
     cpu = acc = None
     if not args.no_cpu_baseline:
         # the oracle over a sample of each list's groups: timed as the CPU baseline and used as the checker
