@@ -512,7 +512,7 @@ def pvel_adjoint(args, torch, S, W):
                   note=NOTE + "; kernel_ms = the adjoint's kernels (pair pass + partial-sum reductions) "
                               "between HIP events on their stream"), "pvel-adjoint-65536x65536", "fp64"),
          roofline_mixed=attach_traffic(roofline("pvel_vjp", "mixed", pairs, adj_mixed_kernel_ms,
-                                                "vjp_pair_kernel<stokes_pvel,MIXED,T=4>"),
+                                                "vjp_pair_kernel<stokes_pvel,MIXED,T=8>"),
                                        "pvel-adjoint-65536x65536", "mixed"),
          cpu_baseline=cpu, accuracy_vs_oracle=acc)
 

@@ -1274,18 +1274,23 @@ extern "C" int stkfmm_p2p_vjp_device(int uker, int precision, int nSrc, const do
     // 12.0 / 11.3 / 11.6 / 10.7 / 11.1 / 10.4 / 10.7 and 7.5 / 8.5 / 7.9 / 8.2 / 7.6 / 7.7 / 7.1 / 7.3
     // -- the blocks fill whole rounds of the resident waves at 16 and 64 (FP64: 4,096 and 16,384
     // blocks on 4,096 wave slots) and leave a part-empty last round at 32 (86 x 86 blocks); the same
-    // order holds for fused PVel and RPY.  T = 1 is slower.  Mixed mode takes T = 4 from 16 tiles of
-    // 256 targets: half the LDS reads and source-sum rotations per pair, 114-118 VGPRs (4 waves per
-    // SIMD); 65,536^2 mixed, same box, T = 2 -> 4 (ms): Stokeslet 6.03 -> 5.52, fused PVel 12.56 ->
-    // 11.99, PVel 6.45 -> 6.32, RPY + Laplacian 8.25 -> 8.16.  FP64 at T = 4 would need 166+ VGPRs.
+    // order holds for fused PVel and RPY.  T = 1 is slower.  Mixed mode takes more targets per lane
+    // once there are 16 tiles of them: fewer LDS reads and source-sum rotations per pair.  65,536^2
+    // mixed, same box, T = 2 -> 4 (ms): Stokeslet 6.03 -> 5.52, fused PVel 12.56 -> 11.99, PVel 6.45 ->
+    // 6.32, RPY + Laplacian 8.25 -> 8.16; T = 4 -> 8 (3 waves per SIMD, value-gradient terms summed
+    // per step): PVel 6.03 -> 5.59, RPY + Laplacian 7.58 -> 7.46, but Stokeslet 5.15 -> 5.26 and fused
+    // PVel 11.39 -> 12.08 (spills at 168 VGPRs), so T = 8 only for the first two.  FP64 at T = 4 would
+    // need 166+ VGPRs.
     VjpLayout l;
     l.T = (nTrg >= 16 * kWave) ? 2 : 1;
-    if (precision == STKFMM_PREC_MIXED && nTrg >= 16 * 4 * kWave)
-        l.T = 4;
-    if (const char *e = std::getenv("STKFMM_VJP_T")) { // tuning and tests: 1, 2, or 4 (mixed)
+    if (precision == STKFMM_PREC_MIXED && nTrg >= 16 * 4 * kWave) {
+        const bool t8 = uker == STKFMM_UKER_STOKES_PVEL || uker == STKFMM_UKER_RPY_ULAPU;
+        l.T = (t8 && nTrg >= 16 * 8 * kWave) ? 8 : 4;
+    }
+    if (const char *e = std::getenv("STKFMM_VJP_T")) { // tuning and tests: 1, 2, or 4 and 8 (mixed)
         const long t = std::strtol(e, nullptr, 10);
-        if (t != 1 && t != 2 && !(t == 4 && precision == STKFMM_PREC_MIXED))
-            return set_error(STKFMM_ERR_ARG, "STKFMM_VJP_T must be 1, 2, or 4 (mixed only) (got %s)", e);
+        if (t != 1 && t != 2 && !((t == 4 || t == 8) && precision == STKFMM_PREC_MIXED))
+            return set_error(STKFMM_ERR_ARG, "STKFMM_VJP_T must be 1, 2, or 4 or 8 (mixed only) (got %s)", e);
         l.T = (int)t;
     }
     const long long ttiles = std::max(1, (nTrg + kWave * l.T - 1) / (kWave * l.T));

@@ -53,8 +53,11 @@ struct VjpPass {
     double *spart;
 };
 
+// eight targets per lane: at least 3 waves per SIMD asked of the compiler (168 VGPRs; left to itself
+// it schedules the 8-target step into ~240, 2 waves per SIMD)
 template <class V, int PREC, int T>
-__global__ void __launch_bounds__(kWave) vjp_pair_kernel(VjpPass a) {
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(T >= 8 ? 3 : 1)))
+vjp_pair_kernel(VjpPass a) {
     using CT = typename std::conditional<PREC == 0, double, float>::type;
     constexpr int KS = V::KS, KT = V::KT, ND = 3 + KS, TT = kWave * T;
     __shared__ double s_x[kSrcTile], s_y[kSrcTile], s_z[kSrcTile];
@@ -120,18 +123,33 @@ __global__ void __launch_bounds__(kWave) vjp_pair_kernel(VjpPass a) {
 #pragma unroll
                 for (int q = 0; q < KS; ++q)
                     f[q] = s_v[q][j];
+                // mixed mode: this step's source-value terms over the lane's T targets are summed
+                // apart and added to the rotating FP32 sums once, so those take 64 additions per target
+                // tile whatever T is.  Added straight in (2 FMAs per component per target), the value
+                // gradients of the 65,536^2 fused PVel adjoint missed the 1e-6 bar at T = 4 (errorMaxRel
+                // 1.3e-6; 2.6e-7 with the sums apart), PVel's at T = 8 (1.4e-6; 4.0e-7).  FP64 adds them
+                // straight in.  -0 is the exact identity of addition (+0 is not: +0 + -0 = +0), so the
+                // first "+= x" into each sum folds to "= x".
+                constexpr int KG = (PREC != 0) ? KS : 1;
+                CT gs[KG];
+#pragma unroll
+                for (int q = 0; q < KG; ++q)
+                    gs[q] = CT(-0.0);
 #pragma unroll
                 for (int t = 0; t < T; ++t) {
-                    // -0 is the exact identity of addition (+0 is not: +0 + -0 = +0), so the
-                    // micro-kernel's first "gr += x" folds to "gr = x" instead of an add per component
                     CT gr[3] = {CT(-0.0), CT(-0.0), CT(-0.0)};
-                    V::template vjp<CT>((CT)(tx[t] - sx), (CT)(ty[t] - sy), (CT)(tz[t] - sz), f, w[t], gr, rs + 3);
+                    V::template vjp<CT>((CT)(tx[t] - sx), (CT)(ty[t] - sy), (CT)(tz[t] - sz), f, w[t], gr,
+                                        (PREC != 0) ? gs : rs + 3);
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {
                         tg[t][d] += gr[d];
                         rs[d] -= gr[d];
                     }
                 }
+                if constexpr (PREC != 0)
+#pragma unroll
+                    for (int q = 0; q < KS; ++q)
+                        rs[3 + q] += gs[q];
 #pragma unroll
                 for (int d = 0; d < ND; ++d)
                     rs[d] = rotate_down(rs[d]);

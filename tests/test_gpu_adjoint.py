@@ -69,20 +69,42 @@ def test_vjp_mixed_four_targets_per_lane(uk):
         assert l2 < 1e-5 and mx < 1e-5, (name, l2, mx)
 
 
+@pytest.mark.parametrize("uk", [0, 2, 6, 7, 8, 9, 10, 11, 12, 13])
+def test_vjp_mixed_eight_targets_per_lane(uk, monkeypatch):
+    # 8 targets per lane (mixed mode's default for PVel and RPY from 8,192 targets; STKFMM_VJP_T for
+    # the rest): 8,300 targets (a ragged last tile of 108) against 650 sources, every micro-kernel
+    # with an adjoint
+    monkeypatch.setenv("STKFMM_VJP_T", "8")
+    say(f"vjp mixed T=8 uker={uk} ns=650 nt=8300")
+    rng = np.random.default_rng(770 + uk)
+    ns, nt = 650, 8300
+    src = rng.random((ns, 3))
+    trg = np.concatenate([src[:13], rng.random((nt - 13, 3))])
+    v = source_values(uk, ns, rng)
+    w = rng.normal(size=(nt, O.trg_dim(uk)))
+    ref = O.direct_vjp(uk, src, v, trg, w)
+    got = S.vjp_device(uk, to_dev(src), to_dev(v), to_dev(trg), to_dev(w), precision=S.PREC.MIXED)
+    for g, r, name in zip(got, ref, ("g_src_value", "g_src_coord", "g_trg_coord")):
+        l2, mx = O.max_error(host(g), r)
+        assert l2 < 1e-5 and mx < 1e-5, (name, l2, mx)
+
+
 def test_vjp_targets_per_lane_override_is_checked(monkeypatch):
-    # STKFMM_VJP_T: 1, 2, or 4 in mixed mode; anything else is refused with an argument error
+    # STKFMM_VJP_T: 1, 2, or 4 and 8 in mixed mode; anything else is refused with an argument error
     rng = np.random.default_rng(760)
     src, trg = to_dev(rng.random((70, 3))), to_dev(rng.random((80, 3)))
     v, w = to_dev(rng.normal(size=(70, 4))), to_dev(rng.normal(size=(80, 4)))
-    for bad, prec in (("3", S.PREC.MIXED), ("4", S.PREC.FP64), ("0", S.PREC.MIXED)):
+    for bad, prec in (("3", S.PREC.MIXED), ("4", S.PREC.FP64), ("8", S.PREC.FP64), ("16", S.PREC.MIXED),
+                      ("0", S.PREC.MIXED)):
         monkeypatch.setenv("STKFMM_VJP_T", bad)
         with pytest.raises(S.ArgumentError):
             S.vjp_device(S.UKER.STOKES_PVEL, src, v, trg, w, precision=prec)
-    monkeypatch.setenv("STKFMM_VJP_T", "4")
-    got = S.vjp_device(S.UKER.STOKES_PVEL, src, v, trg, w, precision=S.PREC.MIXED)
     ref = O.direct_vjp(7, host(src), host(v), host(trg), host(w))
-    for g, r in zip(got, ref):
-        assert O.max_error(host(g), r)[1] < 1e-5
+    for t in ("4", "8"):
+        monkeypatch.setenv("STKFMM_VJP_T", t)
+        got = S.vjp_device(S.UKER.STOKES_PVEL, src, v, trg, w, precision=S.PREC.MIXED)
+        for g, r in zip(got, ref):
+            assert O.max_error(host(g), r)[1] < 1e-5, t
 
 
 def test_autograd_function_matches_vjp():
@@ -193,21 +215,26 @@ def test_workload4_adjoint_full_size():
     assert O.max_error(g_trg[ti], gt_ref)[0] < 1e-11
 
 
-def test_workload4_adjoint_full_size_mixed():
-    """65,536 x 65,536 PVel, mixed precision: every gradient against the oracle's FP64 gradient on
-    sampled sources and targets (the whole source set against sampled targets and vice versa), to the
-    mixed mode's bar, and the sampled entries of the mixed gradient against the FP64 one."""
-    say("workload 4 full size, mixed")
+@pytest.mark.parametrize("uk", [7, 13, 11])
+def test_workload4_adjoint_full_size_mixed(uk):
+    """65,536 x 65,536 (the workload's PVel points; fused PVel and RPY with values of their own),
+    mixed precision at its default targets per lane: every gradient against the oracle's FP64
+    gradient on sampled sources and targets (the whole source set against sampled targets and vice
+    versa), to the mixed mode's bar, and every entry of the mixed gradient against the FP64 one."""
+    say(f"workload 4 full size, mixed, uker={uk}")
     src, val, trg, w = W.adjoint_pvel()
+    if uk != 7:
+        g = np.random.default_rng(45 + uk)
+        val = source_values(uk, len(src), g)
+        w = g.normal(size=(len(trg), O.trg_dim(uk)))
     ds, dv, dt, dw = to_dev(src), to_dev(val), to_dev(trg), to_dev(w)
-    g_val, g_src, g_trg = (host(x) for x in S.vjp_device(S.UKER.STOKES_PVEL, ds, dv, dt, dw,
-                                                          precision=S.PREC.MIXED))
-    f_val, f_src, f_trg = (host(x) for x in S.vjp_device(S.UKER.STOKES_PVEL, ds, dv, dt, dw))
+    g_val, g_src, g_trg = (host(x) for x in S.vjp_device(uk, ds, dv, dt, dw, precision=S.PREC.MIXED))
+    f_val, f_src, f_trg = (host(x) for x in S.vjp_device(uk, ds, dv, dt, dw))
     rng = np.random.default_rng(44)
     si = rng.choice(len(src), 64, replace=False)
     ti = rng.choice(len(trg), 64, replace=False)
-    gv_ref, gs_ref, _ = O.direct_vjp(7, src[si], val[si], trg, w)
-    _, _, gt_ref = O.direct_vjp(7, src, val, trg[ti], w[ti])
+    gv_ref, gs_ref, _ = O.direct_vjp(uk, src[si], val[si], trg, w)
+    _, _, gt_ref = O.direct_vjp(uk, src, val, trg[ti], w[ti])
     for got, ref, what in ((g_val[si], gv_ref, "src_value"), (g_src[si], gs_ref, "src_coord"),
                            (g_trg[ti], gt_ref, "trg_coord")):
         l2, mx = O.max_error(got, ref)
