@@ -121,7 +121,7 @@ template <int UK, int PREC, int T, bool FARB = false>
 __global__ void __launch_bounds__(kWave) STKP2P_ITEM_ATTR
 __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_kernel(ItemArgs a) {
     using K = typename UkerOf<UK>::type;
-    constexpr int KL = K::KL, KT = K::KT, KS = K::KS, KS1 = K::KS1;
+    constexpr int KL = StagedCount<K, PREC>::value, KT = K::KT, KS = K::KS, KS1 = K::KS1;
     constexpr int TT = kWave * T;
 
     // FP64: values as [KL][tile] (two consecutive sources per 16-byte read);

@@ -23,7 +23,7 @@ namespace stkp2p {
 template <class K>
 __device__ __forceinline__ void fp64_item_sums(const ItemArgs &a, const Item &item, double *acc, int jf, int js,
                                                double tx, double ty, double tz, double lam) {
-    constexpr int KS = K::KS, KS1 = K::KS1, KL = K::KL, KT = K::KT;
+    constexpr int KS = K::KS, KS1 = K::KS1, KL = StagedCount<K, 0>::value, KT = K::KT;
 #pragma unroll
     for (int k = 0; k < KT; ++k)
         acc[k] = 0.0;

@@ -32,7 +32,8 @@ namespace stkp2p {
 template <class K, int PREC>
 struct RecLayout {
     // 8-byte words per record: x, y, z, then KL values (FP64: one word each; mixed: two FP32 per word)
-    static constexpr int VW = (PREC == 0) ? K::KL : (K::KL + 1) / 2;
+    static constexpr int KL = StagedCount<K, PREC>::value;
+    static constexpr int VW = (PREC == 0) ? KL : (KL + 1) / 2;
     static constexpr int WORDS = 3 + VW;
 };
 
@@ -44,7 +45,7 @@ template <int UK, int PREC>
 __global__ void __launch_bounds__(256) stage_records_kernel(StageArgs a) {
     using K = typename UkerOf<UK>::type;
     using L = RecLayout<K, PREC>;
-    constexpr int KS = K::KS, KS1 = K::KS1, KL = K::KL;
+    constexpr int KS = K::KS, KS1 = K::KS1, KL = L::KL;
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.n_src)
         return;
@@ -79,7 +80,7 @@ template <int UK, int PREC, int T>
 __global__ void __launch_bounds__(kWave) p2p_items_sgpr_kernel(ItemArgs a, const double *recs, int level_min) {
     using K = typename UkerOf<UK>::type;
     using L = RecLayout<K, PREC>;
-    constexpr int KL = K::KL, KT = K::KT;
+    constexpr int KL = L::KL, KT = K::KT;
     constexpr int TT = kWave * T;
     const int lane = threadIdx.x;
     ItemCursor cur(a);

@@ -61,6 +61,26 @@ __device__ __forceinline__ double rsqrt_f64(double r2) {
         return y;
 }
 
+// 2/r in FP64: rsqrt_f64's last Newton step without its factor 1/2, y (3 - r2 y^2) -- three
+// operations (y^2, the FMA, the product) for its four, for micro-kernels whose staged values absorb
+// the powers of 2.  MASK as rsqrt_f64.
+template <bool MASK = true>
+__device__ __forceinline__ double twice_rsqrt_f64(double r2) {
+    static_assert(STKP2P_FP64_NEWTON >= 1, "the 2/r form is a Newton step");
+    double y = __builtin_amdgcn_rsq(r2);
+    const double hr = 0.5 * r2;
+#pragma unroll
+    for (int i = 1; i < STKP2P_FP64_NEWTON; ++i) {
+        const double e = __builtin_fma(-hr, y * y, 0.5);
+        y = __builtin_fma(y, e, y);
+    }
+    const double w = y * __builtin_fma(-r2, y * y, 3.0);
+    if constexpr (MASK)
+        return (r2 > 0.0) ? w : 0.0;
+    else
+        return w;
+}
+
 // FP32: v_rsq_f32 seed (measured on gfx950: 23.2 correct bits, i.e. within 1.1e-7 relative) and
 // one Newton step.
 template <bool MASK = true>
@@ -111,6 +131,13 @@ template <class K, class = void> struct StagesByPrecision { static constexpr boo
 template <class K> struct StagesByPrecision<K, decltype((void)K::STAGE_BY_PRECISION)> {
     static constexpr bool value = K::STAGE_BY_PRECISION;
 };
+// staged values per source in precision PREC: KL_FP64 for PREC 0 when the micro-kernel declares it
+// (its FP64 form stages more values than its FP32 one), else KL
+template <class K, int PREC, class = void> struct StagedCount { static constexpr int value = K::KL; };
+template <class K> struct StagedCount<K, 0, decltype((void)K::KL_FP64)> {
+    static constexpr int value = K::KL_FP64;
+};
+
 template <class K, int PREC>
 __host__ __device__ inline void stage_for(const double *in, double *o) {
     if constexpr (StagesByPrecision<K>::value)
@@ -403,12 +430,16 @@ struct UkStokesDoublePVel {
 // with cc - TrD formed as one fused multiply-add chain started from the staged constant.  Q'/r^2 as
 // in UkStokesDoublePVel, by precision:
 //   FP32: Q' by Horner reusing dz^2; staged fx, fy, fz, -TrD, g0..g5, TrD + tr D.
-//   FP64: Q'/r^2 = g2 + rest/r^2 with g2 folded into the chain's constant; staged fx, fy, fz,
-//         -TrD + g2, g0 - g2, g1 - g2, g3, g4, g5, TrD + tr D (the last slot unused).
-// FP32: 26 operations per pair besides the displacement and the reciprocal square root; FP64: 25;
-// the two layers' expressions side by side: 30.
+//   FP64: Q'/r^2 = g2 + rest/r^2 with g2 folded into the chain's constant, and w = 2/r from the
+//         Newton step without its 1/2 (twice_rsqrt_f64, one operation fewer): with 1/r = w/2,
+//         1/r^2 = w^2/4 and 1/r^3 = w^3/8, t/8 = (f/8).r + (-TrD + g2)/8 + rest/32 w^2, and then
+//         t/r^3 = (t/8) w^3, p/2 = (t/8 + (TrD + tr D)/8) w^3, f_i/r = (f_i/2) w -- f staged twice
+//         (f/8 for the dot product, f/2 for f/r), every scaling a power of 2 (exact); staged f/8,
+//         (-TrD + g2)/8, (g0 - g2, g1 - g2, g3, g4, g5)/32, (TrD + tr D)/8, f/2 (KL_FP64 = 13).
+// FP32: 26 operations per pair besides the displacement and the reciprocal square root; FP64: 25
+// and a cheaper reciprocal square root; the two layers' expressions side by side: 30.
 struct UkStokesPVelFused {
-    static constexpr int KS = 13, KS1 = 4, KL = 11, KT = 4;
+    static constexpr int KS = 13, KS1 = 4, KL = 11, KL_FP64 = 13, KT = 4;
     static constexpr bool UNMASKED_EVAL = true, STAGE_BY_PRECISION = true;
     static constexpr int MIXED_UNROLL = 1; // item kernel's mixed-mode source loop (measured fastest)
     static constexpr unsigned SRC_SCALE_MASK = 0x1FF8u;
@@ -420,10 +451,12 @@ struct UkStokesPVelFused {
         UkStokesDoublePVel::stage_p<PREC>(in + 4, d);
         o[0] = in[0]; o[1] = in[1]; o[2] = in[2];
         if constexpr (PREC == 0) {
-            o[3] = -in[3] + d[2];
-            o[4] = d[0]; o[5] = d[1]; o[6] = d[3]; o[7] = d[4]; o[8] = d[5];
-            o[9] = in[3] + d[6];
-            o[10] = 0.0;
+            o[0] = 0.125 * in[0]; o[1] = 0.125 * in[1]; o[2] = 0.125 * in[2];
+            o[3] = 0.125 * (-in[3] + d[2]);
+            o[4] = 0.03125 * d[0]; o[5] = 0.03125 * d[1]; o[6] = 0.03125 * d[3]; o[7] = 0.03125 * d[4];
+            o[8] = 0.03125 * d[5];
+            o[9] = 0.125 * (in[3] + d[6]);
+            o[10] = 0.5 * in[0]; o[11] = 0.5 * in[1]; o[12] = 0.5 * in[2];
         } else {
             o[3] = -in[3];
             for (int k = 0; k < 6; ++k)
@@ -435,17 +468,29 @@ struct UkStokesPVelFused {
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         const R z2 = dz * dz;
         R r2 = dx * dx + (dy * dy + z2);
+        R cm = f[0] * dx + (f[1] * dy + (f[2] * dz + f[3]));
+        if constexpr (sizeof(R) == 8) {
+            const R w = twice_rsqrt_f64<M>(r2);
+            const R w2 = w * w;
+            const R w3 = w2 * w;
+            const R a1 = f[4] * dx + f[6] * dy + f[7] * dz;
+            const R a2 = f[5] * dy + f[8] * dz;
+            const R t8 = (dx * a1 + dy * a2) * w2 + cm; // t/8
+            const R b = t8 * w3;                         // t/r^3
+            u[0] += (t8 + f[9]) * w3;
+            u[1] += f[10] * w;
+            u[1] += dx * b;
+            u[2] += f[11] * w;
+            u[2] += dy * b;
+            u[3] += f[12] * w;
+            u[3] += dz * b;
+            return;
+        }
         const auto rs = pair_rsqrt<M>(r2);
         R rinv = rs.s, rinv2 = rs.s2;
         R rinv3 = rinv2 * rinv;
-        R cm = f[0] * dx + (f[1] * dy + (f[2] * dz + f[3]));
         R t, G;
-        if constexpr (sizeof(R) == 8) {
-            const R a1 = f[4] * dx + f[6] * dy + f[7] * dz;
-            const R a2 = f[5] * dy + f[8] * dz;
-            t = (dx * a1 + dy * a2) * rinv2 + cm;
-            G = f[9];
-        } else {
+        {
             const R a1 = f[4] * dx + f[7] * dy + f[8] * dz;
             const R a2 = f[5] * dy + f[9] * dz;
             t = (dx * a1 + (dy * a2 + f[6] * z2)) * rinv2 + cm;

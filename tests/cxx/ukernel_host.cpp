@@ -10,13 +10,14 @@ using namespace stkp2p;
 namespace {
 template <class K, class R, bool MASK = true>
 void pair(const double *r, const double *fin, double scale, double *u) {
-    double vst[K::KL];
+    constexpr int P = sizeof(R) == 8 ? 0 : 1, KL = StagedCount<K, P>::value;
+    double vst[KL];
     double vin[K::KS];
     for (int k = 0; k < K::KS; ++k)
         vin[k] = fin[k];
-    stage_for<K, sizeof(R) == 8 ? 0 : 1>(vin, vst);
-    R f[K::KL];
-    for (int k = 0; k < K::KL; ++k)
+    stage_for<K, P>(vin, vst);
+    R f[KL];
+    for (int k = 0; k < KL; ++k)
         f[k] = (R)vst[k];
     R acc[K::KT];
     for (int k = 0; k < K::KT; ++k)
