@@ -95,6 +95,19 @@ __device__ __forceinline__ float rsqrt_f32(float r2) {
         return y;
 }
 
+// 2/r in FP32: rsqrt_f32's Newton step without its factor 1/2, as twice_rsqrt_f64.
+template <bool MASK = true>
+__device__ __forceinline__ float twice_rsqrt_f32(float r2) {
+    const float y = __builtin_amdgcn_rsqf(r2);
+    const float w = y * __builtin_fmaf(-r2, y * y, 3.0f);
+    if constexpr (MASK)
+        return (r2 > 0.0f) ? w : 0.0f;
+    else
+        return w;
+}
+template <bool M> __device__ __forceinline__ double twice_rsqrt(double r2) { return twice_rsqrt_f64<M>(r2); }
+template <bool M> __device__ __forceinline__ float twice_rsqrt(float r2) { return twice_rsqrt_f32<M>(r2); }
+
 // masked and refined (the adjoint kernels and the rest of the family)
 __device__ __forceinline__ double rsqrt_masked(double r2) { return rsqrt_f64<true>(r2); }
 __device__ __forceinline__ float rsqrt_masked(float r2) { return rsqrt_f32<true>(r2); }
@@ -158,12 +171,12 @@ struct UkLaplaceP {
     static constexpr unsigned SRC_SCALE_MASK = 0u;
     __host__ __device__ static constexpr int deg(int) { return 1; }
     __host__ __device__ static constexpr double post(int k) { return ((void)k, 1.0); }
-    __device__ static void stage(const double *in, double *o) { o[0] = in[0]; }
+    // staged q/2 against w = 2/r (twice_rsqrt, one operation fewer than 1/r)
+    __device__ static void stage(const double *in, double *o) { o[0] = 0.5 * in[0]; }
     template <class R, bool M = true>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
-        R rinv = pair_rsqrt<M>(r2).s;
-        u[0] += f[0] * rinv;
+        u[0] += f[0] * twice_rsqrt<M>(r2);
     }
 };
 
@@ -174,14 +187,17 @@ struct UkLaplaceDipoleP {
     static constexpr unsigned SRC_SCALE_MASK = 0x7u;
     __host__ __device__ static constexpr int deg(int) { return 1; }
     __host__ __device__ static constexpr double post(int k) { return ((void)k, 1.0); }
-    __device__ static void stage(const double *in, double *o) { o[0] = in[0]; o[1] = in[1]; o[2] = in[2]; }
+    // staged n/8 against w = 2/r (twice_rsqrt, one operation fewer than 1/r): (n/8) w^3 = n/r^3
+    __device__ static void stage(const double *in, double *o) {
+        o[0] = 0.125 * in[0]; o[1] = 0.125 * in[1]; o[2] = 0.125 * in[2];
+    }
     template <class R, bool M = true>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
-        const auto rs = pair_rsqrt<M>(r2);
-        R rinv3 = rs.s2 * rs.s;
+        const R w = twice_rsqrt<M>(r2);
+        R w3 = (w * w) * w;
         R rdotn = f[0] * dx + f[1] * dy + f[2] * dz;
-        u[0] += rinv3 * rdotn;
+        u[0] += w3 * rdotn;
     }
 };
 
@@ -193,14 +209,17 @@ struct UkLaplacePGrad {
     static constexpr bool UNMASKED_EVAL = true;
     static constexpr unsigned SRC_SCALE_MASK = 0u;
     __host__ __device__ static constexpr int deg(int k) { return k == 0 ? 1 : 2; }
-    __host__ __device__ static constexpr double post(int k) { return k == 0 ? 1.0 : -1.0; }
-    __device__ static void stage(const double *in, double *o) { o[0] = in[0]; }
+    // the gradient sums hold 4 q r / r^3 (below): -1/4 once per target
+    __host__ __device__ static constexpr double post(int k) { return k == 0 ? 1.0 : -0.25; }
+    __device__ static void stage(const double *in, double *o) { o[0] = 0.5 * in[0]; }
+    // w = 2/r (the Newton step without its 1/2, one operation fewer), staged q/2: q/r = (q/2) w and
+    // (q/r) w^2 = 4 q/r^3 -- every scaling a power of 2
     template <class R, bool M = true>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
-        const auto rs = pair_rsqrt<M>(r2);
-        R qr = f[0] * rs.s;
-        R sv = qr * rs.s2;
+        const R w = twice_rsqrt<M>(r2);
+        R qr = f[0] * w;
+        R sv = qr * (w * w);
         u[0] += qr;
         u[1] += sv * dx;
         u[2] += sv * dy;
@@ -314,17 +333,37 @@ struct UkLaplaceDipolePGradGrad {
 // Stokeslet 3 -> 3 (PVFMM stokes_vel, StokesLayerKernel.hpp:38-42; closed form of
 // Test/SimpleKernel.cpp:986-1023 with a = 0): u = f/r + r (f.r)/r^3, evaluated as
 // u_i = (1/r) (f_i + r_i h) with h = (f.r)/r^2: one multiply per pair fewer than f_i/r + r_i (f.r)/r^3
-// (no 1/r^3), the same two FMAs per component.
+// (no 1/r^3), the same two FMAs per component.  FP64 takes w = 2/r (twice_rsqrt, one operation
+// fewer) with f staged twice: u_i = w (f_i/2 + r_i h'), h' = ((f/8).r) w^2 = (f.r)/(2 r^2) -- every
+// scaling a power of 2; FP32 keeps one staged f (a second would cost an LDS row read per source).
 struct UkStokesVel {
-    static constexpr int KS = 3, KS1 = 3, KL = 3, KT = 3;
-    static constexpr bool UNMASKED_EVAL = true;
+    static constexpr int KS = 3, KS1 = 3, KL = 3, KL_FP64 = 6, KT = 3;
+    static constexpr bool UNMASKED_EVAL = true, STAGE_BY_PRECISION = true;
     static constexpr unsigned SRC_SCALE_MASK = 0u;
     __host__ __device__ static constexpr int deg(int) { return 1; }
     __host__ __device__ static constexpr double post(int k) { return ((void)k, 1.0); }
-    __device__ static void stage(const double *in, double *o) { o[0] = in[0]; o[1] = in[1]; o[2] = in[2]; }
+    template <int PREC>
+    __host__ __device__ static void stage_p(const double *in, double *o) {
+        if constexpr (PREC == 0) {
+            for (int i = 0; i < 3; ++i) {
+                o[i] = 0.5 * in[i];
+                o[3 + i] = 0.125 * in[i];
+            }
+        } else {
+            o[0] = in[0]; o[1] = in[1]; o[2] = in[2];
+        }
+    }
     template <class R, bool M = true>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
+        if constexpr (sizeof(R) == 8) {
+            const R w = twice_rsqrt_f64<M>(r2);
+            R h = (f[3] * dx + f[4] * dy + f[5] * dz) * (w * w);
+            u[0] += w * (f[0] + dx * h);
+            u[1] += w * (f[1] + dy * h);
+            u[2] += w * (f[2] + dz * h);
+            return;
+        }
         const auto rs = pair_rsqrt<M>(r2);
         R rinv = rs.s;
         R h = (f[0] * dx + f[1] * dy + f[2] * dz) * rs.s2;
