@@ -316,3 +316,62 @@ def test_autotuner_chooses_a_candidate_and_caches(tmp_path, monkeypatch):
     plan.execute(d_x, d_sl, d_x, out, src_value2=d_dl)
     l2, mx = O.max_error(host(out), oracle_leaves(tr, 13, xs, sl, dl, None))
     assert l2 < MIXED_TOL and mx < MIXED_TOL
+
+
+def piece_tiles(n, tt, pieces=True):
+    """Target tiles of a group of n targets as the planner lays them out: whole tiles of tt, then the
+    remainder -- cut into power-of-two tiles, largest first, in plans of several groups -- until the
+    rest fills its lane slots (tt >> lane split, at least tt/16) to within 1/8 or a cut saves none."""
+    def slots(c):
+        ls = 0
+        while ls < 4 and (c << (ls + 1)) <= tt:
+            ls += 1
+        return tt >> ls
+    out = [tt] * (n // tt)
+    n %= tt
+    while n > 0:
+        q = 1 << (n.bit_length() - 1)
+        if not pieces or 8 * (slots(n) - n) <= slots(n) or q == n or q + slots(n - q) >= slots(n):
+            out.append(n)
+            break
+        out.append(q)
+        n -= q
+    return out
+
+
+@pytest.mark.parametrize("split", [0, -1], ids=["pieces", "one-lane"])
+@pytest.mark.parametrize("T", [1, 2, 4])
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
+def test_remainder_tiles(prec, T, split):
+    """Groups whose sizes leave every kind of remainder (1, just above a power of two, just below a
+    whole tile, several whole tiles): the tile count the planner reports, and every target against
+    the oracle."""
+    say(f"remainder tiles prec={prec} T={T} split={split}")
+    sizes = [1, 9, 17, 40, 65, 100, 127, 128, 129, 193, 255, 300, 333, 513]
+    g = np.random.default_rng(100 + T)
+    uk = 7
+    n_trg, n_src = sum(sizes), 5000
+    trg = g.random((n_trg, 3))
+    src = g.random((n_src, 3))
+    v = g.normal(size=(n_src, O.src_dim(uk)))
+    gb = np.cumsum([0] + sizes[:-1]).astype(np.int32)
+    ge = (gb + np.array(sizes)).astype(np.int32)
+    rb, re_, gp = [], [], [0]
+    for _ in sizes:
+        for _ in range(2):
+            a = int(g.integers(0, n_src - 600))
+            rb.append(a)
+            re_.append(a + int(g.integers(1, 600)))
+        gp.append(len(rb))
+    work = dict(n_src=n_src, n_trg=n_trg, grp_trg_begin=gb, grp_trg_end=ge, grp_level=None,
+                grp_rng_ptr=np.array(gp, np.int32), rng_begin=np.array(rb, np.int32),
+                rng_end=np.array(re_, np.int32))
+    plan = S.Plan(uk, prec, tune=dict(targets_per_lane=T, split_tiles=split), **work)
+    expect = sum(len(piece_tiles(n, 64 * T, split == 0)) for n in sizes)
+    assert plan.stats()["n_tiles"] == expect
+    out = zeros(n_trg, O.trg_dim(uk))
+    plan.execute(to_dev(src), to_dev(v), to_dev(trg), out)
+    ref = O.ranges(uk, src, v, trg, gb, ge, gp, rb, re_)
+    l2, mx = O.max_error(host(out), ref)
+    tol = FP64_TOL if prec == 0 else MIXED_TOL
+    assert l2 < tol and mx < tol, (l2, mx)
