@@ -421,7 +421,10 @@ __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_ke
                         const bool bfar = FAR && bclear && (fm & bm) == bm;
                         const int j1 = (n - j0 < kMixedAccBlock) ? n : j0 + kMixedAccBlock;
                         run(j0, j1, bclear, bfar);
-                        if (j1 % FL == 0 || j1 == n) {
+                        // (a group of one block flushes unconditionally: j1 is then always a multiple
+                        // of the block or n, and saying so lets the compiler start every block's
+                        // sums from zero instead of carrying them between the forms' loops)
+                        if (FL == kMixedAccBlock || j1 % FL == 0 || j1 == n) {
 #pragma unroll
                             for (int t = 0; t < T; ++t)
 #pragma unroll
