@@ -95,6 +95,14 @@ template <class K> struct MixedUnroll<K, decltype((void)K::MIXED_UNROLL)> {
     static constexpr int value = K::MIXED_UNROLL;
 };
 
+// the mixed-mode source loop takes each block's first source on its own (MIXED_PEEL of the
+// micro-kernel, else not): measured on MI355X, same box, twice each: the fused PVel U-list 7.870 ->
+// 7.830 ms and the RPY U-list 39.98 -> 39.78 ms, but the ordered Stokeslet instances 34.54 -> 34.86 ms
+template <class K, class = void> struct MixedPeel { static constexpr bool value = false; };
+template <class K> struct MixedPeel<K, decltype((void)K::MIXED_PEEL)> {
+    static constexpr bool value = K::MIXED_PEEL;
+};
+
 // minimum resident waves per SIMD asked of the compiler for the mixed-mode item kernel:
 // MIXED_MIN_WAVES of the micro-kernel, else 1 (no constraint).  The RPY U-list kernel asks for 4: the
 // compiler then fits 128 VGPRs (instead of 158, 3 waves) by spilling a few item-level values to
@@ -296,12 +304,13 @@ __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_ke
                 // sources [jb, je), whose FP32 partial sums then join the FP64 accumulators
                 // mixed: FP32 partial sums of the current group of MixedFlush<K> sources, added into the
                 // FP64 accumulators when the group ends
+                // (-0 is the exact additive identity: -0 + x == x for every x, +0 included)
                 float part[T][KT];
 #pragma unroll
                 for (int t = 0; t < T; ++t)
 #pragma unroll
                     for (int k = 0; k < KT; ++k)
-                        part[t][k] = 0.0f;
+                        part[t][k] = -0.0f;
                 auto tile_pairs = [&](auto strided, auto clear, auto farc, int jb, int je) {
                     constexpr bool STR = decltype(strided)::value;
                     constexpr bool MASK = !decltype(clear)::value;
@@ -335,51 +344,58 @@ __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_ke
                                 src(j);
                         }
                     } else {
-                        {
-                            const int j0 = jb, j1 = je;
-                            // sources per iteration: MixedUnroll<K> (measured on MI355X, T = 2: the
-                            // fused PVel U-list 8.37 ms at 1, 8.62 at 2, 8.91 at 4; RPY 1 % faster
-                            // at 2 than at 1)
-#pragma unroll MixedUnroll<K>::value
-                            for (int j = j0 + j_first; j < j1; j += j_step) {
-                                // coordinates first: LDS reads return in issue order, and the
-                                // displacement is needed before the values
-                                float hx, hy, hz, lx, ly, lz;
-                                double sx, sy, sz;
-                                if constexpr (FARM) {
-                                    hx = s_vf[j][FOFF];
-                                    hy = s_vf[j][FOFF + 1];
-                                    hz = s_vf[j][FOFF + 2];
-                                } else if constexpr (HILO) {
-                                    hx = s_vf[j][0];
-                                    hy = s_vf[j][1];
-                                    hz = s_vf[j][2];
-                                    lx = s_vf[j][3];
-                                    ly = s_vf[j][4];
-                                    lz = s_vf[j][5];
-                                } else {
-                                    sx = s_x[j];
-                                    sy = s_y[j];
-                                    sz = s_z[j];
-                                }
-                                float f[KL];
-#pragma unroll
-                                for (int k = 0; k < KL; ++k)
-                                    f[k] = s_vf[j][KOFF + k];
-#pragma unroll
-                                for (int t = 0; t < T; ++t) {
-                                    if constexpr (FARM)
-                                        eval_pair<K, false>(part[t], tfx[t] - hx, tfy[t] - hy, tfz[t] - hz, f);
-                                    else if constexpr (HILO)
-                                        eval_pair<K, MASK>(part[t], (thx[t] - hx) + (tlx[t] - lx),
-                                                                (thy[t] - hy) + (tly[t] - ly),
-                                                                (thz[t] - hz) + (tlz[t] - lz), f);
-                                    else
-                                        eval_pair<K, MASK>(part[t], (float)(tx[t] - sx), (float)(ty[t] - sy),
-                                                           (float)(tz[t] - sz), f);
-                                }
+                        auto src = [&](int j) {
+                            // coordinates first: LDS reads return in issue order, and the
+                            // displacement is needed before the values
+                            float hx, hy, hz, lx, ly, lz;
+                            double sx, sy, sz;
+                            if constexpr (FARM) {
+                                hx = s_vf[j][FOFF];
+                                hy = s_vf[j][FOFF + 1];
+                                hz = s_vf[j][FOFF + 2];
+                            } else if constexpr (HILO) {
+                                hx = s_vf[j][0];
+                                hy = s_vf[j][1];
+                                hz = s_vf[j][2];
+                                lx = s_vf[j][3];
+                                ly = s_vf[j][4];
+                                lz = s_vf[j][5];
+                            } else {
+                                sx = s_x[j];
+                                sy = s_y[j];
+                                sz = s_z[j];
                             }
+                            float f[KL];
+#pragma unroll
+                            for (int k = 0; k < KL; ++k)
+                                f[k] = s_vf[j][KOFF + k];
+#pragma unroll
+                            for (int t = 0; t < T; ++t) {
+                                if constexpr (FARM)
+                                    eval_pair<K, false>(part[t], tfx[t] - hx, tfy[t] - hy, tfz[t] - hz, f);
+                                else if constexpr (HILO)
+                                    eval_pair<K, MASK>(part[t], (thx[t] - hx) + (tlx[t] - lx),
+                                                            (thy[t] - hy) + (tly[t] - ly),
+                                                            (thz[t] - hz) + (tlz[t] - lz), f);
+                                else
+                                    eval_pair<K, MASK>(part[t], (float)(tx[t] - sx), (float)(ty[t] - sy),
+                                                       (float)(tz[t] - sz), f);
+                            }
+                        };
+                        // MixedPeel: the block's first source on its own -- where the sums start the
+                        // block at -0 (groups of one block), its accumulations fold into plain
+                        // products and the sums need no clearing.  Then the rest, MixedUnroll<K>
+                        // sources per iteration (measured on MI355X, T = 2: the fused PVel U-list
+                        // 8.37 ms at 1, 8.62 at 2, 8.91 at 4; RPY 1 % faster at 2 than at 1)
+                        int j = jb + j_first;
+                        if constexpr (MixedPeel<K>::value) {
+                            if (j < je)
+                                src(j);
+                            j += j_step;
                         }
+#pragma unroll MixedUnroll<K>::value
+                        for (; j < je; j += j_step)
+                            src(j);
                     }
                 };
                 using F = std::false_type;
@@ -430,7 +446,7 @@ __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_ke
 #pragma unroll
                                 for (int k = 0; k < KT; ++k) {
                                     acc[t][k] += (double)part[t][k];
-                                    part[t][k] = 0.0f;
+                                    part[t][k] = -0.0f;
                                 }
                         }
                     }

@@ -481,6 +481,7 @@ struct UkStokesPVelFused {
     static constexpr int KS = 13, KS1 = 4, KL = 11, KL_FP64 = 13, KT = 4;
     static constexpr bool UNMASKED_EVAL = true, STAGE_BY_PRECISION = true;
     static constexpr int MIXED_UNROLL = 1; // item kernel's mixed-mode source loop (measured fastest)
+    static constexpr bool MIXED_PEEL = true; // and its first source per block on its own (measured)
     static constexpr unsigned SRC_SCALE_MASK = 0x1FF8u;
     __host__ __device__ static constexpr int deg(int k) { return k == 0 ? 2 : 1; }
     __host__ __device__ static constexpr double post(int k) { return k == 0 ? 2.0 : 1.0; }
@@ -622,6 +623,7 @@ struct UkRpyULapU {
     // outputs costs less; 4M RPY U-list, same box, twice each: 40.68 -> 40.29 ms, errorL2 4.24e-7 ->
     // 3.16e-7, maxrel 3.38e-7 -> 3.10e-7
     static constexpr int MIXED_ACC_SOURCES = 32;
+    static constexpr bool MIXED_PEEL = true; // item kernel: first source per block on its own (measured)
     static constexpr unsigned SRC_SCALE_MASK = 0x8u;
     __host__ __device__ static constexpr int deg(int k) { return k < 3 ? 1 : 3; }
     __host__ __device__ static constexpr double post(int k) { return k < 3 ? 1.0 : -6.0; }
