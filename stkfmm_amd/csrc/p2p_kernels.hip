@@ -32,6 +32,8 @@ __global__ void __launch_bounds__(256) p2p_reduce_kernel(ReduceArgs a) {
     const long long row = tl.trg_begin + i;
     const long long o = a.perm ? (long long)a.perm[row] : row;
     double s = 0.0;
+    // unrolled so that the loads of several slots are in flight together (the adds keep slot order)
+#pragma unroll 4
     for (int slot = tl.slot_begin; slot < tl.slot_end; ++slot)
         s += a.partial[((long long)slot * a.kt + k) * a.tt + i];
     a.out.p[o * a.out.s + k * a.out.fs] += a.scale * s;
@@ -52,6 +54,7 @@ __global__ void __launch_bounds__(256) p2p_reduce_rows_kernel(ReduceArgs a) {
     const long long o = a.perm ? (long long)a.perm[row] : row;
     for (int k = 0; k < a.kt; ++k) {
         double s = 0.0;
+#pragma unroll 4
         for (int slot = tl.slot_begin; slot < tl.slot_end; ++slot)
             s += a.partial[((long long)slot * a.kt + k) * a.tt + i];
         a.out.p[o * a.out.s + k * a.out.fs] += a.scale * s;
