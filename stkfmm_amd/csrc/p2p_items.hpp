@@ -77,6 +77,7 @@ namespace stkp2p {
 #ifndef STKP2P_FP64_UNROLL_T1
 #define STKP2P_FP64_UNROLL_T1 4
 #endif
+
 #ifndef STKP2P_FAR_F32
 #define STKP2P_FAR_F32 1
 #endif
@@ -99,6 +100,15 @@ template <class K> struct MixedUnroll<K, decltype((void)K::MIXED_UNROLL)> {
 // micro-kernel, else not): measured on MI355X, same box, twice each: the fused PVel U-list 7.870 ->
 // 7.830 ms and the RPY U-list 39.98 -> 39.78 ms, but the ordered Stokeslet instances 34.54 -> 34.86 ms
 template <class K, class = void> struct MixedPeel { static constexpr bool value = false; };
+// the mixed-mode source loop at one lane per target fully unrolled over a block, each source at an
+// immediate LDS offset from one base and a scheduling barrier between sources (MIXED_BLOCK_UNROLL of
+// the micro-kernel, else not): measured on MI355X, same box, twice each: the RPY U-list 41.67 ->
+// 40.90 ms (126 VGPRs and no scratch, against 128 and an 8-byte spill), but the fused PVel U-list
+// 8.10 -> 8.58 ms and the ordered Stokeslet instances 36.17 -> 39.3 ms
+template <class K, class = void> struct MixedBlockUnroll { static constexpr bool value = false; };
+template <class K> struct MixedBlockUnroll<K, decltype((void)K::MIXED_BLOCK_UNROLL)> {
+    static constexpr bool value = K::MIXED_BLOCK_UNROLL;
+};
 template <class K> struct MixedPeel<K, decltype((void)K::MIXED_PEEL)> {
     static constexpr bool value = K::MIXED_PEEL;
 };
@@ -388,14 +398,27 @@ __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_ke
                         // sources per iteration (measured on MI355X, T = 2: the fused PVel U-list
                         // 8.37 ms at 1, 8.62 at 2, 8.91 at 4; RPY 1 % faster at 2 than at 1)
                         int j = jb + j_first;
-                        if constexpr (MixedPeel<K>::value) {
-                            if (j < je)
-                                src(j);
-                            j += j_step;
-                        }
+                        if constexpr (!STR && MixedBlockUnroll<K>::value) {
+                            // one lane per target: the block's sources fully unrolled, each at an
+                            // immediate LDS offset from one base (no per-source address or loop
+                            // counter), with a scheduling barrier between sources so that registers
+                            // stay as in the rolled loop
+#pragma unroll
+                            for (int u = 0; u < kMixedAccBlock; ++u) {
+                                if (jb + u < je)
+                                    src(jb + u);
+                                __builtin_amdgcn_sched_barrier(0);
+                            }
+                        } else {
+                            if constexpr (MixedPeel<K>::value) {
+                                if (j < je)
+                                    src(j);
+                                j += j_step;
+                            }
 #pragma unroll MixedUnroll<K>::value
-                        for (; j < je; j += j_step)
-                            src(j);
+                            for (; j < je; j += j_step)
+                                src(j);
+                        }
                     }
                 };
                 using F = std::false_type;

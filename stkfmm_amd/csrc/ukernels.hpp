@@ -624,6 +624,7 @@ struct UkRpyULapU {
     // 3.16e-7, maxrel 3.38e-7 -> 3.10e-7
     static constexpr int MIXED_ACC_SOURCES = 32;
     static constexpr bool MIXED_PEEL = true; // item kernel: first source per block on its own (measured)
+    static constexpr bool MIXED_BLOCK_UNROLL = true; // and its blocks fully unrolled (measured)
     static constexpr unsigned SRC_SCALE_MASK = 0x8u;
     __host__ __device__ static constexpr int deg(int k) { return k < 3 ? 1 : 3; }
     __host__ __device__ static constexpr double post(int k) { return k < 3 ? 1.0 : -6.0; }
