@@ -104,7 +104,8 @@ template <class K, class = void> struct MixedPeel { static constexpr bool value 
 // immediate LDS offset from one base and a scheduling barrier between sources (MIXED_BLOCK_UNROLL of
 // the micro-kernel, else not): measured on MI355X, same box, twice each: the RPY U-list 41.67 ->
 // 40.90 ms (126 VGPRs and no scratch, against 128 and an 8-byte spill), but the fused PVel U-list
-// 8.10 -> 8.58 ms and the ordered Stokeslet instances 36.17 -> 39.3 ms
+// 8.10 -> 8.58 ms and the ordered Stokeslet instances 36.17 -> 39.3 ms (the fused PVel with two
+// sources per iteration and a barrier between them: 7.64 -> 7.92 ms; it keeps the rolled loop)
 template <class K, class = void> struct MixedBlockUnroll { static constexpr bool value = false; };
 template <class K> struct MixedBlockUnroll<K, decltype((void)K::MIXED_BLOCK_UNROLL)> {
     static constexpr bool value = K::MIXED_BLOCK_UNROLL;
