@@ -508,7 +508,7 @@ def pvel_adjoint(args, torch, S, W):
          forward_gpair_s=pairs / fwd / 1e9, adjoint_gpair_s=pairs / adj / 1e9,
          adjoint_mixed_ms=1e3 * adj_mixed, adjoint_mixed_gpair_s=pairs / adj_mixed / 1e9,
          roofline=attach_traffic(
-             dict(roofline("pvel_vjp", "fp64", pairs, adj_kernel_ms, "vjp_pair_kernel<stokes_pvel,FP64,T=2>"),
+             dict(roofline("pvel_vjp", "fp64", pairs, adj_kernel_ms, "vjp_pair_kernel<stokes_pvel,FP64,T=4>"),
                   note=NOTE + "; kernel_ms = the adjoint's kernels (pair pass + partial-sum reductions) "
                               "between HIP events on their stream"), "pvel-adjoint-65536x65536", "fp64"),
          roofline_mixed=attach_traffic(roofline("pvel_vjp", "mixed", pairs, adj_mixed_kernel_ms,

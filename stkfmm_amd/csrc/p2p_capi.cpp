@@ -1312,18 +1312,27 @@ extern "C" int stkfmm_p2p_vjp_device(int uker, int precision, int nSrc, const do
     // mixed, same box, T = 2 -> 4 (ms): Stokeslet 6.03 -> 5.52, fused PVel 12.56 -> 11.99, PVel 6.45 ->
     // 6.32, RPY + Laplacian 8.25 -> 8.16; T = 4 -> 8 (3 waves per SIMD, value-gradient terms summed
     // per step): PVel 6.03 -> 5.59, RPY + Laplacian 7.58 -> 7.46, but Stokeslet 5.15 -> 5.26 and fused
-    // PVel 11.39 -> 12.08 (spills at 168 VGPRs), so T = 8 only for the first two.  FP64 at T = 4 would
-    // need 166+ VGPRs.
+    // PVel 11.39 -> 12.08 (spills at 168 VGPRs), so T = 8 only for the first two.  FP64 takes four
+    // targets per lane as well (3 waves per SIMD at 152-168 VGPRs), except for the double-layer and
+    // fused PVel, which spill there: 65,536^2, same box, T = 2 -> 4 (ms): PVel 8.74 -> 8.35, Stokeslet
+    // 8.45 -> 7.95, LapPGrad 6.29 -> 5.90, RPY + Laplacian 11.88 -> 11.62 (fused PVel 18.1 -> 21.7).
     VjpLayout l;
     l.T = (nTrg >= 16 * kWave) ? 2 : 1;
-    if (precision == STKFMM_PREC_MIXED && nTrg >= 16 * 4 * kWave) {
-        const bool t8 = uker == STKFMM_UKER_STOKES_PVEL || uker == STKFMM_UKER_RPY_ULAPU;
-        l.T = (t8 && nTrg >= 16 * 8 * kWave) ? 8 : 4;
+    if (nTrg >= 16 * 4 * kWave) {
+        if (precision == STKFMM_PREC_MIXED) {
+            const bool t8 = uker == STKFMM_UKER_STOKES_PVEL || uker == STKFMM_UKER_RPY_ULAPU;
+            l.T = (t8 && nTrg >= 16 * 8 * kWave) ? 8 : 4;
+        } else if (vjp_fp64_four_targets(uker)) {
+            l.T = 4;
+        }
     }
-    if (const char *e = std::getenv("STKFMM_VJP_T")) { // tuning and tests: 1, 2, or 4 and 8 (mixed)
+    if (const char *e = std::getenv("STKFMM_VJP_T")) { // tuning and tests: 1, 2, 4, or 8 (mixed)
         const long t = std::strtol(e, nullptr, 10);
-        if (t != 1 && t != 2 && !((t == 4 || t == 8) && precision == STKFMM_PREC_MIXED))
-            return set_error(STKFMM_ERR_ARG, "STKFMM_VJP_T must be 1, 2, or 4 or 8 (mixed only) (got %s)", e);
+        const bool t4 = t == 4 && (precision == STKFMM_PREC_MIXED || vjp_fp64_four_targets(uker));
+        if (t != 1 && t != 2 && !t4 && !(t == 8 && precision == STKFMM_PREC_MIXED))
+            return set_error(STKFMM_ERR_ARG,
+                             "STKFMM_VJP_T must be 1, 2, 4 (FP64: not for the double-layer or fused PVel) or 8 "
+                             "(mixed only) (got %s)", e);
         l.T = (int)t;
     }
     const long long ttiles = std::max(1, (nTrg + kWave * l.T - 1) / (kWave * l.T));

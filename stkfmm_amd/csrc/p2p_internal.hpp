@@ -216,6 +216,11 @@ int uker_ks1(int uk);
 int uker_kt(int uk);
 double uker_scale(int uk);
 bool uker_has_vjp(int uk);
+// the FP64 adjoint of micro-kernel uk runs four targets per lane (3 waves per SIMD, no spill):
+// measured on MI355X, 65,536^2, four targets against two: 3.7-7.6 % faster for the Laplace, Stokeslet,
+// RegVel, single-layer PVel and RPY kernels; the double-layer PVel spills and measured the same, the
+// fused PVel spills and measured 20 % slower, so those two keep two
+constexpr bool vjp_fp64_four_targets(int uk) { return uk != 8 && uk != 13; }
 
 // launchers (host); return hipSuccess or the launch error.  launch_items / items_occupancy also
 // take prec = kPrecMixedFar: the mixed-mode LDS item kernel with the far-tile displacement (T = 2)

@@ -233,12 +233,12 @@ static hipError_t launch_vjp_t(const VjpArgs &a, const VjpLayout &l, double *par
 
 template <int UK, int PREC>
 static hipError_t launch_vjp_p(const VjpArgs &a, const VjpLayout &l, double *partial, hipStream_t s) {
-    if constexpr (PREC != 0) { // four or eight targets per lane: mixed mode only (FP64 would lose occupancy)
+    if constexpr (PREC != 0 || vjp_fp64_four_targets(UK)) // four targets per lane
         if (l.T == 4)
             return launch_vjp_t<UK, PREC, 4>(a, l, partial, s);
+    if constexpr (PREC != 0) // eight: mixed mode only
         if (l.T == 8)
             return launch_vjp_t<UK, PREC, 8>(a, l, partial, s);
-    }
     return (l.T == 1) ? launch_vjp_t<UK, PREC, 1>(a, l, partial, s) : launch_vjp_t<UK, PREC, 2>(a, l, partial, s);
 }
 
@@ -248,7 +248,8 @@ bool uker_has_vjp(int uk) {
 }
 
 hipError_t launch_vjp(int uk, int prec, const VjpArgs &a, const VjpLayout &l, double *partial, hipStream_t s) {
-    if (l.T != 1 && l.T != 2 && !((l.T == 4 || l.T == 8) && prec != 0))
+    if (l.T != 1 && l.T != 2 && !(l.T == 4 && (prec != 0 || vjp_fp64_four_targets(uk))) &&
+        !(l.T == 8 && prec != 0))
         return hipErrorInvalidValue;
     switch (uk) {
     case 0: return prec ? launch_vjp_p<0, 1>(a, l, partial, s) : launch_vjp_p<0, 0>(a, l, partial, s);

@@ -56,7 +56,7 @@ struct VjpPass {
 // eight targets per lane: at least 3 waves per SIMD asked of the compiler (168 VGPRs; left to itself
 // it schedules the 8-target step into ~240, 2 waves per SIMD)
 template <class V, int PREC, int T>
-__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(T >= 8 ? 3 : 1)))
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu((T >= 8 || (PREC == 0 && T >= 4)) ? 3 : 1)))
 vjp_pair_kernel(VjpPass a) {
     using CT = typename std::conditional<PREC == 0, double, float>::type;
     constexpr int KS = V::KS, KT = V::KT, ND = 3 + KS, TT = kWave * T;

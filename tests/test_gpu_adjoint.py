@@ -6,7 +6,7 @@ import stkfmm_amd as S
 from oracle import oracle as O
 from stkfmm_amd import workloads as W
 from tests.conftest import source_values
-from tests.gpu_util import MIXED_TOL, host, say, to_dev, zeros
+from tests.gpu_util import FP64_TOL, MIXED_TOL, host, say, to_dev, zeros
 
 pytestmark = pytest.mark.gpu
 
@@ -69,6 +69,24 @@ def test_vjp_mixed_four_targets_per_lane(uk):
         assert l2 < 1e-5 and mx < 1e-5, (name, l2, mx)
 
 
+@pytest.mark.parametrize("uk", [0, 2, 6, 7, 9, 10, 11, 12])
+def test_vjp_fp64_four_targets_per_lane(uk):
+    # FP64 takes 4 targets per lane from 4,096 targets for these micro-kernels: 4,133 targets (a ragged
+    # last tile of 37, coincident points included) against 700 sources, to the FP64 bar
+    say(f"vjp fp64 T=4 uker={uk} ns=700 nt=4133")
+    rng = np.random.default_rng(790 + uk)
+    ns, nt = 700, 4133
+    src = rng.random((ns, 3))
+    trg = np.concatenate([src[:11], rng.random((nt - 11, 3))])
+    v = source_values(uk, ns, rng)
+    w = rng.normal(size=(nt, O.trg_dim(uk)))
+    ref = O.direct_vjp(uk, src, v, trg, w)
+    got = S.vjp_device(uk, to_dev(src), to_dev(v), to_dev(trg), to_dev(w), precision=S.PREC.FP64)
+    for g, r, name in zip(got, ref, ("g_src_value", "g_src_coord", "g_trg_coord")):
+        l2, mx = O.max_error(host(g), r)
+        assert l2 < FP64_TOL and mx < FP64_TOL, (name, l2, mx)
+
+
 @pytest.mark.parametrize("uk", [0, 2, 6, 7, 8, 9, 10, 11, 12, 13])
 def test_vjp_mixed_eight_targets_per_lane(uk, monkeypatch):
     # 8 targets per lane (mixed mode's default for PVel and RPY from 8,192 targets; STKFMM_VJP_T for
@@ -90,15 +108,20 @@ def test_vjp_mixed_eight_targets_per_lane(uk, monkeypatch):
 
 
 def test_vjp_targets_per_lane_override_is_checked(monkeypatch):
-    # STKFMM_VJP_T: 1, 2, or 4 and 8 in mixed mode; anything else is refused with an argument error
+    # STKFMM_VJP_T: 1, 2, 4 (FP64: not for the double-layer or fused PVel) and 8 in mixed mode;
+    # anything else is refused with an argument error
     rng = np.random.default_rng(760)
     src, trg = to_dev(rng.random((70, 3))), to_dev(rng.random((80, 3)))
     v, w = to_dev(rng.normal(size=(70, 4))), to_dev(rng.normal(size=(80, 4)))
-    for bad, prec in (("3", S.PREC.MIXED), ("4", S.PREC.FP64), ("8", S.PREC.FP64), ("16", S.PREC.MIXED),
-                      ("0", S.PREC.MIXED)):
+    for bad, prec in (("3", S.PREC.MIXED), ("8", S.PREC.FP64), ("16", S.PREC.MIXED), ("0", S.PREC.MIXED)):
         monkeypatch.setenv("STKFMM_VJP_T", bad)
         with pytest.raises(S.ArgumentError):
             S.vjp_device(S.UKER.STOKES_PVEL, src, v, trg, w, precision=prec)
+    v13 = to_dev(rng.normal(size=(70, 13)))
+    for uk, vv in ((S.UKER.STOKES_PVEL_FUSED, v13), (S.UKER.STOKES_DOUBLEPVEL, to_dev(rng.normal(size=(70, 9))))):
+        monkeypatch.setenv("STKFMM_VJP_T", "4")
+        with pytest.raises(S.ArgumentError):
+            S.vjp_device(uk, src, vv, trg, w, precision=S.PREC.FP64)
     ref = O.direct_vjp(7, host(src), host(v), host(trg), host(w))
     for t in ("4", "8"):
         monkeypatch.setenv("STKFMM_VJP_T", t)
