@@ -491,7 +491,7 @@ __attribute__((amdgpu_waves_per_eu(ItemMinWaves<UK, PREC>::value))) p2p_items_ke
         double *dst = a.partial + (long long)item.slot * KT * TT;
 #pragma unroll
         for (int k = 0; k < KT; ++k) {
-            const double osc = ((PREC == 0) ? 1.0 : ldexp(1.0, K::deg(k) * item.level)) * K::post(k);
+            const double osc = ((PREC == 0) ? 1.0 : ldexp(1.0, K::deg(k) * item.level)) * post_for<K, PREC>(k);
 #pragma unroll
             for (int t = 0; t < T; ++t) {
                 const int i = slot0 + tstride * t;

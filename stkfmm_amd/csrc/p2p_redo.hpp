@@ -45,6 +45,12 @@ __device__ __forceinline__ void fp64_item_sums(const ItemArgs &a, const Item &it
             eval_pair<K, true>(acc, tx - sx, ty - sy, tz - sz, vst);
         }
     }
+    // the caller scales by the mixed form's post(); where the FP64 form accumulates an output at
+    // another power of 2, bring it to the mixed form's (exact)
+#pragma unroll
+    for (int k = 0; k < KT; ++k)
+        if constexpr (PostsByPrecision<K>::value)
+            acc[k] *= post_for<K, 0>(k) / post_for<K, 1>(k);
 }
 
 // the end-of-item test and the FP64 path for the targets that need it

@@ -151,6 +151,20 @@ template <class K> struct StagedCount<K, 0, decltype((void)K::KL_FP64)> {
     static constexpr int value = K::KL_FP64;
 };
 
+// constant factor of output k in precision PREC: post_p<PREC>(k) for micro-kernels that declare
+// POST_BY_PRECISION (their FP64 form accumulates some outputs at another power of 2), else post(k)
+template <class K, class = void> struct PostsByPrecision { static constexpr bool value = false; };
+template <class K> struct PostsByPrecision<K, decltype((void)K::POST_BY_PRECISION)> {
+    static constexpr bool value = K::POST_BY_PRECISION;
+};
+template <class K, int PREC>
+__host__ __device__ constexpr double post_for(int k) {
+    if constexpr (PostsByPrecision<K>::value)
+        return K::template post_p<PREC>(k);
+    else
+        return K::post(k);
+}
+
 template <class K, int PREC>
 __host__ __device__ inline void stage_for(const double *in, double *o) {
     if constexpr (StagesByPrecision<K>::value)
@@ -581,19 +595,48 @@ struct UkStokesRegVel {
 //       = f_i (1/r + beta/r^3) + r_i ((f.r)/r^3 - 3 beta (f.r)/r^5),  beta = one_over_three a^2.
 // Staged: fx, fy, fz, beta, -3 beta.
 struct UkRpyU {
-    static constexpr int KS = 4, KS1 = 4, KL = 5, KT = 3;
-    static constexpr bool UNMASKED_EVAL = true;
+    static constexpr int KS = 4, KS1 = 4, KL = 5, KL_FP64 = 8, KT = 3;
+    static constexpr bool UNMASKED_EVAL = true, STAGE_BY_PRECISION = true;
     static constexpr unsigned SRC_SCALE_MASK = 0x8u;
     __host__ __device__ static constexpr int deg(int) { return 1; }
     __host__ __device__ static constexpr double post(int k) { return ((void)k, 1.0); }
-    __device__ static void stage(const double *in, double *o) {
-        o[0] = in[0]; o[1] = in[1]; o[2] = in[2];
-        o[3] = STKP2P_RPY_ONE_OVER_THREE * in[3] * in[3];
-        o[4] = -3.0 * o[3];
+    // FP32: f, c = a^2/3, -3c.  FP64 (w = 2/r from twice_rsqrt, one operation fewer than 1/r): f/2,
+    // f/8, c/4, -3c/4 -- (f/8).r w^3 = (f.r)/r^3, (c/4) w^3 + w = 2 (c/r^3 + 1/r), so f/2 times it is
+    // f (c/r^3 + 1/r), and (-3c/4) g3 w^2 = -3c g3/r^2; every scaling a power of 2
+    template <int PREC>
+    __host__ __device__ static void stage_p(const double *in, double *o) {
+        const double c = STKP2P_RPY_ONE_OVER_THREE * in[3] * in[3];
+        if constexpr (PREC == 0) {
+            for (int i = 0; i < 3; ++i) {
+                o[i] = 0.5 * in[i];
+                o[3 + i] = 0.125 * in[i];
+            }
+            o[6] = 0.25 * c;
+            o[7] = 0.25 * (-3.0 * c);
+        } else {
+            o[0] = in[0]; o[1] = in[1]; o[2] = in[2];
+            o[3] = c;
+            o[4] = -3.0 * c;
+        }
     }
     template <class R, bool M = true>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
+        if constexpr (sizeof(R) == 8) {
+            const R w = twice_rsqrt_f64<M>(r2);
+            const R w2 = w * w;
+            const R w3 = w2 * w;
+            const R g3 = (f[3] * dx + f[4] * dy + f[5] * dz) * w3;
+            const R A2 = f[6] * w3 + w; // 2 A
+            const R B = f[7] * (g3 * w2) + g3;
+            u[0] += f[0] * A2;
+            u[0] += dx * B;
+            u[1] += f[1] * A2;
+            u[1] += dy * B;
+            u[2] += f[2] * A2;
+            u[2] += dz * B;
+            return;
+        }
         const auto rs = pair_rsqrt<M>(r2);
         R rinv = rs.s, rinv2 = rs.s2;
         R rinv3 = rinv2 * rinv;
@@ -616,8 +659,8 @@ struct UkRpyU {
 // with f' = -f/3 staged (g3 = (f.r)/r^3, g5 = (f.r)/r^5; post() applies the -6).  Staged as UkRpyU,
 // then -f/3 (the three extra values fill the 16-byte LDS row that holds the first five).
 struct UkRpyULapU {
-    static constexpr int KS = 4, KS1 = 4, KL = 8, KT = 6;
-    static constexpr bool UNMASKED_EVAL = true;
+    static constexpr int KS = 4, KS1 = 4, KL = 8, KL_FP64 = 11, KT = 6;
+    static constexpr bool UNMASKED_EVAL = true, STAGE_BY_PRECISION = true, POST_BY_PRECISION = true;
     static constexpr int MIXED_MIN_WAVES = 4; // item kernel's mixed-mode occupancy request (measured)
     // FP32 partial sums over 32 sources (the form is still chosen per 16): the FP64 flush of the six
     // outputs costs less; 4M RPY U-list, same box, twice each: 40.68 -> 40.29 ms, errorL2 4.24e-7 ->
@@ -628,15 +671,47 @@ struct UkRpyULapU {
     static constexpr unsigned SRC_SCALE_MASK = 0x8u;
     __host__ __device__ static constexpr int deg(int k) { return k < 3 ? 1 : 3; }
     __host__ __device__ static constexpr double post(int k) { return k < 3 ? 1.0 : -6.0; }
-    __device__ static void stage(const double *in, double *o) {
-        UkRpyU::stage(in, o);
-        o[5] = in[0] * (-1.0 / 3.0);
-        o[6] = in[1] * (-1.0 / 3.0);
-        o[7] = in[2] * (-1.0 / 3.0);
+    // the FP64 form accumulates the Laplacian at 4 times its value (below): -6/4 there
+    template <int PREC>
+    __host__ __device__ static constexpr double post_p(int k) {
+        return k < 3 ? 1.0 : (PREC == 0 ? -1.5 : -6.0);
+    }
+    // staged as UkRpyU in each precision, then f' = -f/3 (FP64: f'/2)
+    template <int PREC>
+    __host__ __device__ static void stage_p(const double *in, double *o) {
+        UkRpyU::stage_p<PREC>(in, o);
+        constexpr int O = PREC == 0 ? 8 : 5;
+        constexpr double S = PREC == 0 ? 0.5 : 1.0;
+        for (int i = 0; i < 3; ++i)
+            o[O + i] = S * (in[i] * (-1.0 / 3.0));
     }
     template <class R, bool M = true>
     __device__ static void eval(R *u, R dx, R dy, R dz, const R *f) {
         R r2 = dx * dx + dy * dy + dz * dz;
+        if constexpr (sizeof(R) == 8) {
+            // UkRpyU's FP64 form; g5' = g3 w^2 = 4 g5, and (f'/2) w^3 = 4 f'/r^3: u3..5 hold 4 times
+            // their value (post_p<0>)
+            const R w = twice_rsqrt_f64<M>(r2);
+            const R w2 = w * w;
+            const R w3 = w2 * w;
+            const R g3 = (f[3] * dx + f[4] * dy + f[5] * dz) * w3;
+            const R g5q = g3 * w2;
+            const R A2 = f[6] * w3 + w;
+            const R B = f[7] * g5q + g3;
+            u[0] += f[0] * A2;
+            u[0] += dx * B;
+            u[1] += f[1] * A2;
+            u[1] += dy * B;
+            u[2] += f[2] * A2;
+            u[2] += dz * B;
+            u[3] += f[8] * w3;
+            u[3] += dx * g5q;
+            u[4] += f[9] * w3;
+            u[4] += dy * g5q;
+            u[5] += f[10] * w3;
+            u[5] += dz * g5q;
+            return;
+        }
         const auto rs = pair_rsqrt<M>(r2);
         R rinv = rs.s, rinv2 = rs.s2;
         R rinv3 = rinv2 * rinv;

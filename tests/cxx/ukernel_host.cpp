@@ -27,7 +27,7 @@ void pair(const double *r, const double *fin, double scale, double *u) {
     else
         K::template eval<R>(acc, (R)r[0], (R)r[1], (R)r[2], f);
     for (int k = 0; k < K::KT; ++k)
-        u[k] = (double)acc[k] * K::post(k) * scale;
+        u[k] = (double)acc[k] * post_for<K, P>(k) * scale;
 }
 
 template <class K>
