@@ -98,7 +98,8 @@ template <class K> struct MixedUnroll<K, decltype((void)K::MIXED_UNROLL)> {
 
 // the mixed-mode source loop takes each block's first source on its own (MIXED_PEEL of the
 // micro-kernel, else not): measured on MI355X, same box, twice each: the fused PVel U-list 7.870 ->
-// 7.830 ms and the RPY U-list 39.98 -> 39.78 ms, but the ordered Stokeslet instances 34.54 -> 34.86 ms
+// 7.830 ms, the RPY U-list 39.98 -> 39.78 ms, the single-layer PVel FMM lists 6.84 -> 6.80 ms (X
+// list -1.3 %, W list +0.4 %), but the ordered Stokeslet instances 34.54 -> 34.86 ms
 template <class K, class = void> struct MixedPeel { static constexpr bool value = false; };
 // the mixed-mode source loop at one lane per target fully unrolled over a block, each source at an
 // immediate LDS offset from one base and a scheduling barrier between sources (MIXED_BLOCK_UNROLL of

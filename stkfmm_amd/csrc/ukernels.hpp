@@ -393,6 +393,7 @@ struct UkStokesVel {
 struct UkStokesPVel {
     static constexpr int KS = 4, KS1 = 4, KL = 4, KT = 4;
     static constexpr bool UNMASKED_EVAL = true;
+    static constexpr bool MIXED_PEEL = true; // item kernel: first source per block on its own (measured)
     static constexpr unsigned SRC_SCALE_MASK = 0x8u;
     __host__ __device__ static constexpr int deg(int k) { return k == 0 ? 2 : 1; }
     __host__ __device__ static constexpr double post(int k) { return k == 0 ? 2.0 : 1.0; }
