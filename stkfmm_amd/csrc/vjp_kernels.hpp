@@ -188,6 +188,8 @@ __global__ void vjp_reduce_kernel(const double *partial, int n, int nd, int nchu
     if (i >= n || out == nullptr || d >= d1)
         return;
     double s = 0.0;
+    // unrolled so that the loads of several chunks are in flight together (the adds keep chunk order)
+#pragma unroll 8
     for (int c = 0; c < nchunk; ++c)
         s += partial[((long long)c * nd + d) * n + i];
     out[(long long)i * ld + off + (d - d0)] = scale * s;
