@@ -1307,7 +1307,9 @@ extern "C" int stkfmm_p2p_vjp_device(int uker, int precision, int nSrc, const do
     // 12.0 / 11.3 / 11.6 / 10.7 / 11.1 / 10.4 / 10.7 and 7.5 / 8.5 / 7.9 / 8.2 / 7.6 / 7.7 / 7.1 / 7.3
     // -- the blocks fill whole rounds of the resident waves at 16 and 64 (FP64: 4,096 and 16,384
     // blocks on 4,096 wave slots) and leave a part-empty last round at 32 (86 x 86 blocks); the same
-    // order holds for fused PVel and RPY.  T = 1 is slower.  Mixed mode takes more targets per lane
+    // order holds for fused PVel and RPY (and again at four / eight targets per lane: 24 / 36 / 48 /
+    // 64 / 96 waves per CU, PVel 5.90 / 5.86 / 6.23 / 5.47 / 5.60 ms mixed, 8.64 / 8.74 / 8.89 / 8.26
+    // / 8.66 FP64).  T = 1 is slower.  Mixed mode takes more targets per lane
     // once there are 16 tiles of them: fewer LDS reads and source-sum rotations per pair.  65,536^2
     // mixed, same box, T = 2 -> 4 (ms): Stokeslet 6.03 -> 5.52, fused PVel 12.56 -> 11.99, PVel 6.45 ->
     // 6.32, RPY + Laplacian 8.25 -> 8.16; T = 4 -> 8 (3 waves per SIMD, value-gradient terms summed
