@@ -20,6 +20,9 @@ Other workloads (--workload), each printing its own JSON line:
                         W-list (M2T) and X-list (S2L) surface evaluations, order-8 surfaces
   wall-images           StkWallFMM's no-slip wall image system (Stokes and RPY), 65,536 sources and
                         targets above the wall, by direct sums, FP64
+  family-ulist          the rest of the kernel family (PVelGrad, PVelLaplacian, Traction single and
+                        double layer, RegFTVelOmega and its sub-kernels, LapQPGradGrad and QuadP) over
+                        the U-list of a 262,144-point tree of the default distribution, mixed and FP64
 
   python bench.py [--workload W] [--steps K] [--warmup W] [--gpus 1]
 """
@@ -51,7 +54,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="pvel-ulist",
                     choices=["pvel-ulist", "pvel-ulist-4m", "lapgrad-direct", "rpy-ulist", "pvel-adjoint",
-                             "stokeslet-instances", "pvel-fmm-lists", "wall-images"])
+                             "stokeslet-instances", "pvel-fmm-lists", "wall-images", "family-ulist"])
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU work budget of the baseline sample")
@@ -784,6 +787,101 @@ def stokeslet_instances(args, torch, S, W):
          variants=res, roofline=rl, cpu_baseline=cpu, accuracy_vs_oracle=acc)
 
 
+# micro-kernels of the family workload; the last source value of 20 and 24 is a regularisation length
+FAMILY = (14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24)
+FAMILY_RADIUS = (20, 24)
+# FP64 FLOPs per pair of each family micro-kernel, counted by the VALU counters on its FP64 item kernel
+# (64 x (ADD + MUL + TRANS + 2 FMA) / pairs; displacement, masking and accumulation included;
+# profiles/family_ulist_262144_fp64_flops_counters.md); the family line's roofline takes them as the
+# algorithmic count
+FAMILY_FLOPS = {14: 119, 15: 106, 16: 60, 17: 71, 18: 57, 19: 101, 20: 113, 21: 54, 22: 124, 23: 47, 24: 65}
+
+
+def family_ulist(args, torch, S, W):
+    """The rest of the kernel family over one U-list: a 262,144-point tree of the default workload's
+    distribution, leaf cap 500, one plan per micro-kernel in each precision (default tuning), every
+    plan timed as the headline's (HIP events on the item kernel, wall clock per step), and each
+    one's result against the oracle on 8 sampled leaves (the CPU baseline times that oracle)."""
+    dev = torch.device("cuda:0")
+    x = W.pvel_points(262_144)
+    tree = S.Tree(x, 500)
+    xs = x[tree.perm]
+    d_x = torch.from_numpy(xs).to(dev)
+    stream = torch.cuda.Stream()
+    rng = np.random.default_rng(31)
+    sel = np.sort(rng.choice(tree.n_leaves, 8, replace=False))
+    rows = np.concatenate([np.arange(tree.leaf_begin[l], tree.leaf_end[l]) for l in sel])
+    sel_pairs = 0
+    for l in sel:
+        ns = int(np.sum(tree.rng_end[tree.ulist_ptr[l]:tree.ulist_ptr[l + 1]]
+                        - tree.rng_begin[tree.ulist_ptr[l]:tree.ulist_ptr[l + 1]]))
+        sel_pairs += ns * int(tree.leaf_end[l] - tree.leaf_begin[l])
+    order = ["mixed", "fp64"] if args.only not in ("mixed", "fp64") else [args.only]
+    kernels, tot_ms = {}, {name: 0.0 for name in order}
+    cpu_s = cpu_pairs = 0.0
+    for uk in FAMILY:
+        ks, kt = S.uker_dims(uk)
+        v = rng.uniform(-1.0, 1.0, (tree.n, ks))
+        if uk in FAMILY_RADIUS:
+            v[:, -1] = np.abs(v[:, -1]) * 0.01
+        d_v = torch.from_numpy(v).to(dev)
+        out = torch.zeros((tree.n, kt), dtype=torch.float64, device=dev)
+        kw = dict(src_coord=d_x, src_value=d_v, trg_coord=d_x, trg_value=out)
+        r, res = {}, {}
+        for name in order:
+            prec = S.PREC.MIXED if name == "mixed" else S.PREC.FP64
+            plan = S.Plan.from_tree(tree, uk, prec)
+            wall, item_ms, reduce_ms, _ = Timed(torch, plan, kw, stream)(args.steps, args.warmup)
+            o = torch.zeros_like(out)
+            plan.execute(stream=stream, **dict(kw, trg_value=o))
+            torch.cuda.synchronize()
+            res[name] = o.cpu().numpy()
+            r[name] = dict(gpair_s=tree.pairs * args.steps / wall / 1e9, ms_per_step=1e3 * wall / args.steps,
+                           item_ms=item_ms, reduce_ms=reduce_ms)
+            tot_ms[name] += 1e3 * wall / args.steps
+            del plan
+        if not args.no_cpu_baseline:
+            from oracle import oracle as O
+            t = time.perf_counter()
+            ref = O.ranges(uk, xs, v, xs, tree.leaf_begin, tree.leaf_end, tree.ulist_ptr, tree.rng_begin,
+                           tree.rng_end, sel=sel, nthreads=threads())
+            cpu_s += time.perf_counter() - t
+            cpu_pairs += sel_pairs
+            for name in order:
+                l2, mx = O.max_error(res[name][rows], ref[rows])
+                r[name]["errorL2"], r[name]["errorMaxRel"] = l2, mx
+        kernels[S.UKER(uk).name.lower()] = dict(uker=uk, **r)
+        print(f"# family-ulist {S.UKER(uk).name.lower()}: " +
+              ", ".join(f"{n} {r[n]['gpair_s']:.1f} Gpair/s ({r[n]['item_ms']:.3f} ms)" for n in order),
+              file=sys.stderr, flush=True)
+    head = order[0]
+    # the dominant kernel: the family member with the longest item kernel in the head precision
+    dom = max(kernels.values(), key=lambda k: k[head]["item_ms"])
+    uk = dom["uker"]
+    rl = None
+    if uk in FAMILY_FLOPS:
+        f = FAMILY_FLOPS[uk]
+        FLOPS["family"] = (f, 3 + S.uker_dims(uk)[1])
+        rl = roofline("family", head, tree.pairs, dom[head]["item_ms"],
+                      f"p2p_items_kernel<{uk},{'MIXED' if head == 'mixed' else 'FP64'},T=2"
+                      f"{',far-tile form' if head == 'mixed' else ''}>")
+        rl["note"] = NOTE + "; FLOPs per pair counted by the VALU counters on the FP64 kernel (FMA = 2)"
+    cpu = None
+    if cpu_s > 0:
+        cpu = {"value": cpu_pairs / cpu_s / 1e9, "unit": "Gpair/s", "cores": threads(), "kind": "port",
+               "sample": (f"8 sampled leaves ({sel_pairs:.3e} pairs) per micro-kernel, all {len(FAMILY)}; the "
+                          "oracle's scalar FP64 restatement of each reference uKerEval, OpenMP over targets"),
+               **host_info()}
+    n_k = len(FAMILY)
+    line(f"Gpair-interactions/s rest of the kernel family over an adaptive-octree U-list ({head})",
+         n_k * tree.pairs / (tot_ms[head] * 1e-3) / 1e9, "Gpair/s", args, tot_ms[head] / n_k,
+         "f64+f32 (mixed)" if head == "mixed" else "f64",
+         {"workload": "family-ulist-262144-clustered-leafcap500", "points": tree.n, "leaves": tree.n_leaves,
+          "pairs_per_kernel": tree.pairs, "kernels": n_k, "precision": head},
+         kernels=kernels, roofline=rl, cpu_baseline=cpu,
+         aggregate_gpair_s={n: n_k * tree.pairs / (tot_ms[n] * 1e-3) / 1e9 for n in order})
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -828,6 +926,8 @@ def main():
         pvel_fmm_lists(args, torch, S, W)
     elif args.workload == "wall-images":
         wall_images(args, torch, S, W)
+    elif args.workload == "family-ulist":
+        family_ulist(args, torch, S, W)
     else:
         stokeslet_instances(args, torch, S, W)
 
