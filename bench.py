@@ -866,6 +866,7 @@ def family_ulist(args, torch, S, W):
                       f"p2p_items_kernel<{uk},{'MIXED' if head == 'mixed' else 'FP64'},T=2"
                       f"{',far-tile form' if head == 'mixed' else ''}>")
         rl["note"] = NOTE + "; FLOPs per pair counted by the VALU counters on the FP64 kernel (FMA = 2)"
+        attach_traffic(rl, "family-ulist-262144-clustered-leafcap500", f"uk{uk}-{head}")
     cpu = None
     if cpu_s > 0:
         cpu = {"value": cpu_pairs / cpu_s / 1e9, "unit": "Gpair/s", "cores": threads(), "kind": "port",
