@@ -862,9 +862,12 @@ def family_ulist(args, torch, S, W):
     if uk in FAMILY_FLOPS:
         f = FAMILY_FLOPS[uk]
         FLOPS["family"] = (f, 3 + S.uker_dims(uk)[1])
+        # the library's default targets per lane: 1 in mixed mode for the family members with many
+        # outputs (the plain item kernel), else 2 (mixed: the far-tile instantiation)
+        t1 = head == "mixed" and uk in (14, 15, 17, 19, 20, 22)
         rl = roofline("family", head, tree.pairs, dom[head]["item_ms"],
-                      f"p2p_items_kernel<{uk},{'MIXED' if head == 'mixed' else 'FP64'},T=2"
-                      f"{',far-tile form' if head == 'mixed' else ''}>")
+                      f"p2p_items_kernel<{uk},{'MIXED' if head == 'mixed' else 'FP64'},T={1 if t1 else 2}"
+                      f"{',far-tile form' if head == 'mixed' and not t1 else ''}>")
         rl["note"] = NOTE + "; FLOPs per pair counted by the VALU counters on the FP64 kernel (FMA = 2)"
         attach_traffic(rl, "family-ulist-262144-clustered-leafcap500", f"uk{uk}-{head}")
     cpu = None

@@ -372,7 +372,16 @@ extern "C" int stkfmm_plan_create(stkfmm_plan **plan_out, int uk, int precision,
     stkfmm_tune tn{};
     if (tune)
         tn = *tune;
-    const int T = tn.targets_per_lane ? tn.targets_per_lane : 2;
+    // default targets per lane: 2, and 1 in mixed mode for the family members with many outputs or
+    // much per-pair state, whose two-target loops run at lower occupancy: 262,144-point U-list, item
+    // kernel + reduce (ms), T = 2 -> 1: PVelGrad 2.25 -> 2.03, LapQPGradGrad 2.34 -> 2.13,
+    // DoubleLaplacian 1.48 -> 1.40 (DoublePVelGrad, DoubleTraction and RegFTVelOmega 9-12 % faster at
+    // T = 1 with the autotuner's item sizes; the others the same or slower at 1)
+    const bool t1 = precision == STKFMM_PREC_MIXED &&
+                    (uk == STKFMM_UKER_STOKES_PVELGRAD || uk == STKFMM_UKER_STOKES_DOUBLEPVELGRAD ||
+                     uk == STKFMM_UKER_STOKES_DOUBLELAPLACIAN || uk == STKFMM_UKER_STOKES_DOUBLETRACTION ||
+                     uk == STKFMM_UKER_STOKES_REGFTVELOMEGA || uk == STKFMM_UKER_LAPLACE_QUADPGRADGRAD);
+    const int T = tn.targets_per_lane ? tn.targets_per_lane : (t1 ? 1 : 2);
     if (T != 1 && T != 2 && T != 4)
         return set_error(STKFMM_ERR_ARG, "targets_per_lane must be 1, 2 or 4 (got %d)", T);
     const int persistent = tn.persistent ? tn.persistent : 1;
