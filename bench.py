@@ -530,7 +530,8 @@ def pvel_adjoint(args, torch, S, W):
 def pvel_fmm_lists(args, torch, S, W):
     """S2M, L2T, W-list and X-list surface evaluations of PVel (the stokes_pvel sub-kernel of each,
     StokesLayerKernel.hpp:44-50) on the 1M-point tree, order-8 surfaces (296 points): one plan per
-    list, mixed and FP64, default tuning."""
+    list, mixed and FP64, tune="auto": S2M and L2T (one signature) take the shipped tuning of
+    stkfmm_amd/tuned_defaults.json, W and X the default (within 1 % of the best candidate)."""
     dev = torch.device("cuda:0")
     t0 = time.perf_counter()
     x = W.pvel_points()
@@ -569,7 +570,7 @@ def pvel_fmm_lists(args, torch, S, W):
     per, results = {}, {}
     for name, (mk, dev_args, _, n_trg, _) in lists.items():
         for prec_name, prec in (("mixed", S.PREC.MIXED), ("fp64", S.PREC.FP64)):
-            plan = mk(uk, prec)
+            plan = mk(uk, prec, tune="auto")
             out = torch.zeros((n_trg, 4), dtype=torch.float64, device=dev)
             kw = dict(src_coord=dev_args[0], src_value=dev_args[1], trg_coord=dev_args[2], trg_value=out)
             wall, item_ms, reduce_ms, _ = Timed(torch, plan, kw, stream)(max(args.steps, 10), args.warmup)
