@@ -4,7 +4,8 @@
 //       PGradGrad single and double layer, Stokeslet, PVel single/double/fused, regularised
 //       Stokeslet, RPY; the rest of the family (14..24) is instantiated in p2p_family.hip.
 //   p2p_reduce_kernel  per target: sum of its tile's partial slots in slot order (deterministic),
-//       times the scale factor, added (+=) into the caller's output (optionally permuted).
+//       times the scale factor, added (+=) into the caller's output (optionally permuted); four
+//       output components per thread for kernels with more than four.
 //   vjp kernels        adjoint of a direct sum: one pass over the pairs gives the target-position,
 //                      source-position and source-value gradients (vjp_kernels.hpp).
 //   The launch entries below dispatch through the per-micro-kernel UkerOps table.
@@ -21,22 +22,42 @@ namespace stkp2p {
 
 
 // out(perm[i], k) += scale * sum over the tile's slots of partial[slot][k][i]; one workgroup per
-// (tile, output component k = blockIdx.y), blockDim = 64*T.  Each (target, k) sums its slots in slot
-// order, so the result does not depend on the grid shape.
+// (tile, group of KG output components k = KG*blockIdx.y + c), blockDim = 64*T.  Each (target, k) sums
+// its slots in slot order, so the result does not depend on the grid shape.  KG = 4 for kernels with
+// more than 4 outputs: the components of a group are summed side by side, so that their loads are in
+// flight together (one workgroup per component left the 16-output kernels with ~10^5 one-wave
+// workgroups, latency-bound: reduce 0.178 -> 0.141 ms on a 262,144-point U-list); KG = 1 otherwise,
+// where grouping left too few workgroups (the 4-output PVel U-list reduce 0.106 -> 0.119 ms).
+template <int KG>
 __global__ void __launch_bounds__(256) p2p_reduce_kernel(ReduceArgs a) {
     const Tile tl = a.tiles[blockIdx.x];
     const int i = threadIdx.x;
-    const int k = blockIdx.y;
     if (i >= tl.trg_count)
         return;
+    const int k0 = blockIdx.y * KG;
+    const int nk = min(KG, a.kt - k0);
     const long long row = tl.trg_begin + i;
     const long long o = a.perm ? (long long)a.perm[row] : row;
-    double s = 0.0;
+    const long long step = (long long)a.kt * a.tt;
+    const double *__restrict__ part = a.partial + (long long)k0 * a.tt + i;
+    double sum[KG];
+#pragma unroll
+    for (int c = 0; c < KG; ++c)
+        sum[c] = 0.0;
     // unrolled so that the loads of several slots are in flight together (the adds keep slot order)
-#pragma unroll 4
-    for (int slot = tl.slot_begin; slot < tl.slot_end; ++slot)
-        s += a.partial[((long long)slot * a.kt + k) * a.tt + i];
-    a.out.p[o * a.out.s + k * a.out.fs] += a.scale * s;
+    constexpr int kSlotUnroll = KG == 1 ? 4 : 2;
+#pragma unroll kSlotUnroll
+    for (int slot = tl.slot_begin; slot < tl.slot_end; ++slot) {
+        const double *__restrict__ q = part + slot * step;
+#pragma unroll
+        for (int c = 0; c < KG; ++c)
+            if (c < nk)
+                sum[c] += q[(long long)c * a.tt];
+    }
+#pragma unroll
+    for (int c = 0; c < KG; ++c)
+        if (c < nk)
+            a.out.p[o * a.out.s + (k0 + c) * a.out.fs] += a.scale * sum[c];
 }
 
 // out(perm[i], k) += scale * (slot sums) for every k of target i in one thread: the same sums in the
@@ -149,7 +170,10 @@ int record_words(int uk, int prec) {
 hipError_t launch_reduce(const ReduceArgs &a, hipStream_t s) {
     if (a.n_tiles <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(p2p_reduce_kernel, dim3(a.n_tiles, a.kt), dim3(a.tt), 0, s, a);
+    if (a.kt > 4)
+        hipLaunchKernelGGL(p2p_reduce_kernel<4>, dim3(a.n_tiles, (a.kt + 3) / 4), dim3(a.tt), 0, s, a);
+    else
+        hipLaunchKernelGGL(p2p_reduce_kernel<1>, dim3(a.n_tiles, a.kt), dim3(a.tt), 0, s, a);
     return hipGetLastError();
 }
 
