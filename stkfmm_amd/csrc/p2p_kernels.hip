@@ -24,10 +24,11 @@ namespace stkp2p {
 // out(perm[i], k) += scale * sum over the tile's slots of partial[slot][k][i]; one workgroup per
 // (tile, group of KG output components k = KG*blockIdx.y + c), blockDim = 64*T.  Each (target, k) sums
 // its slots in slot order, so the result does not depend on the grid shape.  KG = 4 for kernels with
-// more than 4 outputs: the components of a group are summed side by side, so that their loads are in
-// flight together (one workgroup per component left the 16-output kernels with ~10^5 one-wave
-// workgroups, latency-bound: reduce 0.178 -> 0.141 ms on a 262,144-point U-list); KG = 1 otherwise,
-// where grouping left too few workgroups (the 4-output PVel U-list reduce 0.106 -> 0.119 ms).
+// more than 4 outputs over at least 2048 tiles: the components of a group are summed side by side, so
+// that their loads are in flight together (one workgroup per component left the 16-output kernels
+// with ~10^5 one-wave workgroups, latency-bound: reduce 0.178 -> 0.141 ms on a 262,144-point
+// U-list); KG = 1 otherwise, where grouping left too few workgroups (the 4-output PVel U-list reduce
+// 0.106 -> 0.119 ms; 65,536-target direct sums of 512 tiles and ~64 slots each slower).
 template <int KG>
 __global__ void __launch_bounds__(256) p2p_reduce_kernel(ReduceArgs a) {
     const Tile tl = a.tiles[blockIdx.x];
@@ -170,7 +171,7 @@ int record_words(int uk, int prec) {
 hipError_t launch_reduce(const ReduceArgs &a, hipStream_t s) {
     if (a.n_tiles <= 0)
         return hipSuccess;
-    if (a.kt > 4)
+    if (a.kt > 4 && a.n_tiles >= 2048)
         hipLaunchKernelGGL(p2p_reduce_kernel<4>, dim3(a.n_tiles, (a.kt + 3) / 4), dim3(a.tt), 0, s, a);
     else
         hipLaunchKernelGGL(p2p_reduce_kernel<1>, dim3(a.n_tiles, a.kt), dim3(a.tt), 0, s, a);
